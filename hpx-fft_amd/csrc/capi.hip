@@ -1,0 +1,699 @@
+// capi.hip -- plans, twiddle tables, launch sequencing and the C ABI of
+// include/hpxfft_hip.h.
+//
+// A plan is the GPU image of hpxfft::shared::loop after initialize
+// (core/src/shared/loop.cpp:158-189): dimensions, the chosen 1-D kernels
+// (instead of FFTW plans), device scratch (instead of the transpose buffer
+// trans_values_vec_, which the fused column pass does not need at full
+// size), twiddle tables, timing events and the measurement map.
+#include "launch.hpp"
+#include "../../include/hpxfft_hip.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+using namespace hpxfft_host;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                                  \
+    do                                                                                                             \
+    {                                                                                                              \
+        hipError_t e_ = (x);                                                                                       \
+        if (e_ != hipSuccess)                                                                                      \
+        {                                                                                                          \
+            if (e_ == hipErrorOutOfMemory || e_ == hipErrorMemoryAllocation)                                       \
+                return fail(HPXFFT_OUT_OF_MEMORY, std::string(#x) + ": " + hipGetErrorString(e_));                \
+            return fail(HPXFFT_HIP_ERROR, std::string(#x) + ": " + hipGetErrorString(e_));                         \
+        }                                                                                                          \
+    } while (0)
+
+bool is_pow2(size_t v) { return v && !(v & (v - 1)); }
+int ilog2(size_t v)
+{
+    int r = 0;
+    while ((size_t(1) << r) < v) ++r;
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// Twiddle tables: for every N = 2^k, k = 0..16, the entries
+// w_N^m = exp(-2 pi i m / N), m < N, stored at [N, 2N) of one array.  Each
+// value is evaluated directly (no recurrence) in long double after reducing
+// 8m/N to the first octant, so axis and diagonal values are exact.  The FP32
+// table is the FP64 table rounded (the only FP64 step of the FP32 path).
+constexpr int TW_MAX_LG = 16;
+
+void twiddle_ld(uint64_t m, uint64_t n, long double &c, long double &s)
+{
+    m %= n;
+    const long double PI_L = 3.141592653589793238462643383279502884L;
+    const uint64_t m8 = 8 * m, oct = m8 / n, rem = m8 - oct * n;
+    const bool odd = oct & 1;
+    const long double t = (long double)(odd ? n - rem : rem) / (long double)n * (PI_L / 4.0L);
+    const long double ct = rem == 0 && !odd ? 1.0L : cosl(t);
+    const long double st = rem == 0 && !odd ? 0.0L : sinl(t);
+    switch (oct)
+    {
+    case 0: c = ct; s = st; break;
+    case 1: c = st; s = ct; break;
+    case 2: c = -st; s = ct; break;
+    case 3: c = -ct; s = st; break;
+    case 4: c = -ct; s = -st; break;
+    case 5: c = -st; s = -ct; break;
+    case 6: c = st; s = -ct; break;
+    default: c = ct; s = -st; break;
+    }
+}
+
+struct dev_tables
+{
+    double *tw64 = nullptr;   // complex double pairs
+    float *tw32 = nullptr;    // complex float pairs
+};
+
+std::mutex g_tab_mu;
+std::map<int, dev_tables> g_tables;
+
+int get_tables(int dev, dev_tables &out)
+{
+    std::lock_guard<std::mutex> lk(g_tab_mu);
+    auto it = g_tables.find(dev);
+    if (it != g_tables.end())
+    {
+        out = it->second;
+        return HPXFFT_OK;
+    }
+    const size_t n = size_t(2) << TW_MAX_LG;   // entries
+    std::vector<double> h64(2 * n, 0.0);
+    std::vector<float> h32(2 * n, 0.0f);
+    for (int k = 0; k <= TW_MAX_LG; ++k)
+    {
+        const uint64_t N = uint64_t(1) << k;
+        for (uint64_t m = 0; m < N; ++m)
+        {
+            long double c, s;
+            twiddle_ld(m, N, c, s);
+            const size_t idx = N + m;
+            h64[2 * idx] = (double)c;
+            h64[2 * idx + 1] = -(double)s;
+            h32[2 * idx] = (float)h64[2 * idx];
+            h32[2 * idx + 1] = (float)h64[2 * idx + 1];
+        }
+    }
+    dev_tables t;
+    HIPCHK(hipMalloc(&t.tw64, sizeof(double) * 2 * n));
+    HIPCHK(hipMalloc(&t.tw32, sizeof(float) * 2 * n));
+    HIPCHK(hipMemcpy(t.tw64, h64.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t.tw32, h32.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+    g_tables[dev] = t;
+    out = t;
+    return HPXFFT_OK;
+}
+
+struct device_guard
+{
+    int prev = -1;
+    explicit device_guard(int dev)
+    {
+        if (hipGetDevice(&prev) == hipSuccess && prev != dev) (void)hipSetDevice(dev);
+        else prev = -1;
+    }
+    ~device_guard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+int check_device(int &dev)
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0) return fail(HPXFFT_NO_DEVICE, "no HIP device available");
+    HIPCHK(hipGetDevice(&dev));
+    return HPXFFT_OK;
+}
+
+size_t storage_bytes(int prec) { return prec == P_F64 ? 8 : prec == P_F32 ? 4 : 2; }
+size_t compute_bytes(int prec) { return prec == P_F64 ? 8 : 4; }
+const char *prec_name(int prec)
+{
+    switch (prec)
+    {
+    case P_F64: return "f64";
+    case P_F32: return "f32";
+    case P_F16: return "f16-storage/f32-compute";
+    default: return "bf16-storage/f32-compute";
+    }
+}
+
+}   // namespace
+
+// ---------------------------------------------------------------------------
+struct hpxfft_plan_s
+{
+    // problem
+    size_t nx = 0, ny = 0, batch = 1, ld_in = 0, dist_in = 0, ld_out = 0, dist_out = 0;
+    int prec = P_F64, effort = 0;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int lgx = 0, lgL = 0;
+    size_t ncols = 0;
+    // column strategy
+    bool four_step = false;
+    int lgP = 0, lgQ = 0;
+    int W = 8;
+    size_t CW = 0;   // chunk width (columns) of the four-step scratch
+    // device memory
+    dev_tables tabs;
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    int *mid_exp = nullptr;       // 16-bit: per-row exponents after the row pass
+    unsigned *colmax = nullptr;   // 16-bit four-step: per-column max
+    int *col_exp_own = nullptr;   // 16-bit: output exponents when the caller gives none
+    void *adj_work = nullptr;     // adjoint: column-pass result
+    size_t adj_bytes = 0;
+    void *host_buf = nullptr;     // drop-in host path: device copy of the vector_2d
+    size_t host_bytes = 0;
+    // timing
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool ran = false;
+    double h2d = 0.0, d2h = 0.0;
+    double plan_seconds = 0.0, plan_flops = 0.0;
+    std::map<std::string, double> extra;
+    // batched (uneven) plans
+    bool batched = false;
+    std::vector<size_t> bnx, bny;
+    std::vector<hpxfft_plan> sub;   // per-grid plans
+};
+
+namespace {
+
+const void *tw_ptr(const hpxfft_plan_s *p, size_t N)
+{
+    if (p->prec == P_F64) return p->tabs.tw64 + 2 * N;
+    return p->tabs.tw32 + 2 * N;
+}
+
+std::string describe(const hpxfft_plan_s *p)
+{
+    std::ostringstream os;
+    if (p->batched)
+    {
+        os << "hpxfft-hip batched plan: " << p->sub.size() << " grids, precision " << prec_name(p->prec) << "\n";
+        for (size_t i = 0; i < p->sub.size(); ++i)
+            os << "  grid " << i << ": " << p->bnx[i] << " x " << p->bny[i] << "\n";
+        return os.str();
+    }
+    const geom rg = row_geom(p->prec, p->lgL);
+    os << "hpxfft-hip plan (gfx950): Nx=" << p->nx << " Ny=" << p->ny << " batch=" << p->batch << " precision "
+       << prec_name(p->prec) << " effort " << p->effort << "\n";
+    os << "  row r2c kernel: " << (size_t(1) << p->lgL) << "-point complex Stockham per row + r2c split, "
+       << rg.nseq << " row(s)/workgroup, " << rg.nt << " threads, " << rg.E << " elements/thread, radix "
+       << (rg.E < 16 ? rg.E : 16) << (rg.split ? ", LDS exchange in re/im halves" : "") << "\n";
+    if (!p->four_step)
+    {
+        const geom cg = col_geom(p->prec, p->lgx);
+        os << "  column c2c kernel: " << p->nx << "-point, one tile of " << cg.nseq << " adjacent columns x all rows, "
+           << cg.nt << " threads, transposes fused into row-segment loads/stores"
+           << (cg.split ? ", LDS exchange in re/im halves" : "") << "\n";
+    }
+    else
+    {
+        const geom ga = col_geom(p->prec, p->lgQ), gb = col_geom(p->prec, p->lgP);
+        os << "  column c2c four-step Nx = P*Q = " << (1 << p->lgP) << " * " << (1 << p->lgQ) << ", chunks of "
+           << p->CW << " columns (scratch " << p->scratch_bytes << " B)\n";
+        os << "    phase A: " << (1 << p->lgQ) << "-point DFTs over rows m1+P*m2, twiddle w_Nx^(m1 k2), tiles of "
+           << ga.nseq << " columns, " << ga.nt << " threads\n";
+        os << "    phase B: " << (1 << p->lgP) << "-point DFTs, to rows Q*k1+k2, tiles of " << gb.nseq << " columns, "
+           << gb.nt << " threads\n";
+    }
+    const double sr = (double)storage_bytes(p->prec);
+    const double H = (double)p->nx * (double)p->ncols * 2.0 * sr;
+    const double B = ((double)p->nx * p->ny * sr + H) + 2.0 * H;
+    os << "  algorithmic bytes per grid (2-pass model): " << (long long)B << "\n";
+    os << "  plan_flops (2.5 Ny log2 Ny per row + 5 Nx log2 Nx per column): " << p->plan_flops << "\n";
+    return os.str();
+}
+
+int alloc_events(hpxfft_plan_s *p)
+{
+    for (auto &e : p->ev)
+        if (!e) HIPCHK(hipEventCreate(&e));
+    return HPXFFT_OK;
+}
+
+// column pass on the spectrum `spec` (in place), or from `spec_in` to
+// `spec_out` for the adjoint; `conj` = inverse transform.
+int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj, int *col_exp)
+{
+    const size_t sb = storage_bytes(p->prec);
+    const int W = p->W;
+    ColArgs a{};
+    a.pitch = (long long)p->ld_out;
+    a.dist = (long long)p->dist_out;
+    a.ncols = (int)p->ncols;
+    a.row_exp = p->mid_exp;
+    a.out_exp = col_exp;
+    a.rexp_dist = (long long)p->nx;
+    a.cexp_dist = (long long)p->ncols;
+    if (!p->four_step)
+    {
+        a.in = spec_in;
+        a.out = spec_out;
+        a.c0 = 0;
+        a.CW = 0;
+        a.P = 1;
+        a.Q = (int)p->nx;
+        a.tw_L = tw_ptr(p, p->nx);
+        a.tw_N = nullptr;
+        const unsigned gx = (unsigned)((p->ncols + W - 1) / W);
+        HIPCHK(col(p->prec, p->lgx, 0, conj, a, gx, 1, (unsigned)p->batch, p->stream));
+        return HPXFFT_OK;
+    }
+    const int P = 1 << p->lgP, Q = 1 << p->lgQ;
+    a.P = P;
+    a.Q = Q;
+    a.CW = (int)p->CW;
+    a.tw_N = tw_ptr(p, p->nx);
+    if (p->colmax)
+    {
+        const long long n = (long long)(p->ncols * p->batch);
+        hipLaunchKernelGGL(hpxfft_dev::zero_u32_kernel<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
+                           p->colmax, n);
+        HIPCHK(hipGetLastError());
+        a.colmax = p->colmax;
+    }
+    for (size_t b = 0; b < p->batch; ++b)
+    {
+        const char *in_b = reinterpret_cast<const char *>(spec_in) + b * p->dist_out * 2 * sb;
+        char *out_b = reinterpret_cast<char *>(spec_out) + b * p->dist_out * 2 * sb;
+        ColArgs ab = a;
+        ab.dist = 0;
+        if (ab.row_exp) ab.row_exp += b * p->nx;
+        if (ab.out_exp) ab.out_exp += b * p->ncols;
+        if (ab.colmax) ab.colmax += b * p->ncols;
+        for (size_t c0 = 0; c0 < p->ncols; c0 += p->CW)
+        {
+            const size_t cols = std::min(p->CW, p->ncols - c0);
+            const unsigned gx = (unsigned)((cols + W - 1) / W);
+            ab.c0 = (int)c0;
+            // phase A: spectrum -> scratch
+            ab.in = in_b;
+            ab.out = p->scratch;
+            ab.tw_L = tw_ptr(p, (size_t)Q);
+            HIPCHK(col(p->prec, p->lgQ, 1, conj, ab, gx, (unsigned)P, 1, p->stream));
+            // phase B: scratch -> spectrum
+            ab.in = p->scratch;
+            ab.out = out_b;
+            ab.tw_L = tw_ptr(p, (size_t)P);
+            HIPCHK(col(p->prec, p->lgP, 2, conj, ab, gx, (unsigned)Q, 1, p->stream));
+        }
+    }
+    return HPXFFT_OK;
+}
+
+int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_exp, int *col_exp)
+{
+    if (!d_in || !d_out) return fail(HPXFFT_INVALID_ARG, "null device pointer");
+    if (d_in == d_out && (p->ld_in != 2 * p->ld_out || p->dist_in != 2 * p->dist_out))
+        return fail(HPXFFT_INVALID_ARG, "in-place exec needs ld_in == 2*ld_out and dist_in == 2*dist_out");
+    const bool s16 = p->prec == P_F16 || p->prec == P_BF16;
+    if (s16 && !col_exp) col_exp = p->col_exp_own;
+    device_guard g(p->device);
+    if (int rc = alloc_events(p)) return rc;
+    HIPCHK(hipEventRecord(p->ev[0], p->stream));
+    RowArgs r{};
+    r.in = d_in;
+    r.out = d_out;
+    r.ld_in = (long long)p->ld_in;
+    r.ld_out = (long long)(2 * p->ld_out);
+    r.dist_in = (long long)p->dist_in;
+    r.dist_out = (long long)(2 * p->dist_out);
+    r.nrows = (int)p->nx;
+    r.tw = tw_ptr(p, p->ny);
+    r.in_exp = s16 ? in_exp : nullptr;
+    r.out_exp = s16 ? p->mid_exp : nullptr;
+    r.exp_dist = (long long)p->nx;
+    const geom rg = row_geom(p->prec, p->lgL);
+    const unsigned gx = (unsigned)((p->nx + rg.nseq - 1) / rg.nseq);
+    HIPCHK(row_r2c(p->prec, p->lgL, r, gx, (unsigned)p->batch, p->stream));
+    HIPCHK(hipEventRecord(p->ev[1], p->stream));
+    if (int rc = run_columns(p, d_out, d_out, false, s16 ? col_exp : nullptr)) return rc;
+    HIPCHK(hipEventRecord(p->ev[2], p->stream));
+    p->ran = true;
+    p->h2d = p->d2h = 0.0;
+    return HPXFFT_OK;
+}
+
+int plan_common(hpxfft_plan_s *p)
+{
+    if (!is_pow2(p->nx) || !is_pow2(p->ny) || p->ny < 2)
+        return fail(HPXFFT_UNSUPPORTED_SIZE, "Nx and Ny must be powers of two with Ny >= 2 (got " +
+                                                 std::to_string(p->nx) + " x " + std::to_string(p->ny) + ")");
+    p->lgx = ilog2(p->nx);
+    p->lgL = ilog2(p->ny / 2);
+    if (p->lgL > ROW_MAX_LG || p->lgx > 2 * COL_TILE_MAX_LG)
+        return fail(HPXFFT_UNSUPPORTED_SIZE, "grid too large: Nx <= 65536 and Ny <= 32768 are supported");
+    p->ncols = p->ny / 2 + 1;
+    const size_t sb = storage_bytes(p->prec);
+    if (p->ld_in < p->ny || p->ld_out < p->ncols || (p->batch > 1 && (p->dist_in < p->nx * p->ld_in ||
+                                                                      p->dist_out < p->nx * p->ld_out)))
+        return fail(HPXFFT_INVALID_ARG, "layout: need ld_in >= Ny, ld_out >= Ny/2+1 and non-overlapping grids");
+    if (p->ld_in % 2 != 0)
+        return fail(HPXFFT_INVALID_ARG, "ld_in must be even (rows are read as complex pairs)");
+    if (p->dist_in % 2 != 0) return fail(HPXFFT_INVALID_ARG, "dist_in must be even");
+    if (int rc = check_device(p->device)) return rc;
+    if (int rc = get_tables(p->device, p->tabs)) return rc;
+    p->W = col_geom(p->prec, p->lgx).nseq;
+    const int dmax = col_direct_max_lg(p->prec);
+    p->four_step = p->lgx > dmax;
+    if (p->four_step)
+    {
+        p->lgP = p->lgx / 2;
+        p->lgQ = p->lgx - p->lgP;
+        if (p->lgP < COL_TILE_MIN_LG) { p->lgP = COL_TILE_MIN_LG; p->lgQ = p->lgx - p->lgP; }
+        if (p->lgQ > COL_TILE_MAX_LG || p->lgP > COL_TILE_MAX_LG)
+            return fail(HPXFFT_UNSUPPORTED_SIZE, "Nx too large");
+        // default chunk: scratch of about 128 MiB, a whole number of tiles
+        const size_t cb = compute_bytes(p->prec) * 2;
+        size_t cw = (size_t(128) << 20) / (p->nx * cb);
+        cw = std::max<size_t>(cw / p->W * p->W, (size_t)p->W);
+        const size_t full = (p->ncols + p->W - 1) / p->W * p->W;
+        p->CW = std::min(cw, full);
+        p->scratch_bytes = p->nx * p->CW * cb;
+        HIPCHK(hipMalloc(&p->scratch, p->scratch_bytes));
+    }
+    if (p->prec == P_F16 || p->prec == P_BF16)
+    {
+        HIPCHK(hipMalloc(&p->mid_exp, sizeof(int) * p->nx * p->batch));
+        HIPCHK(hipMalloc(&p->col_exp_own, sizeof(int) * p->ncols * p->batch));
+        if (p->four_step) HIPCHK(hipMalloc(&p->colmax, sizeof(unsigned) * p->ncols * p->batch));
+    }
+    (void)sb;
+    if (int rc = alloc_events(p)) return rc;
+    const double lx = (double)p->lgx, ly = (double)ilog2(p->ny);
+    p->plan_flops = (double)p->batch * ((double)p->nx * 2.5 * (double)p->ny * ly +
+                                        (double)p->ncols * 5.0 * (double)p->nx * lx);
+    return HPXFFT_OK;
+}
+
+void free_plan(hpxfft_plan_s *p)
+{
+    if (!p) return;
+    device_guard g(p->device);
+    for (auto s : p->sub) free_plan(s);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    else (void)hipDeviceSynchronize();
+    for (auto &e : p->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipFree(p->scratch);
+    (void)hipFree(p->mid_exp);
+    (void)hipFree(p->colmax);
+    (void)hipFree(p->col_exp_own);
+    (void)hipFree(p->adj_work);
+    (void)hipFree(p->host_buf);
+    delete p;
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return (double)ms * 1e-3;
+}
+
+}   // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+extern "C" {
+
+const char *hpxfft_last_error(void) { return g_last_error.c_str(); }
+
+const char *hpxfft_version(void) { return "hpxfft-hip 0.1 (gfx950)"; }
+
+int hpxfft_plan_flag_from_string(const char *flag, int *effort)
+{
+    if (!flag || !effort) return fail(HPXFFT_INVALID_ARG, "null argument");
+    const std::string f(flag);
+    if (f == "estimate") *effort = HPXFFT_ESTIMATE;
+    else if (f == "measure") *effort = HPXFFT_MEASURE;
+    else if (f == "patient") *effort = HPXFFT_PATIENT;
+    else if (f == "exhaustive") *effort = HPXFFT_EXHAUSTIVE;
+    else return fail(HPXFFT_INVALID_ARG, "Invalid FFTW plan flag string");
+    return HPXFFT_OK;
+}
+
+int hpxfft_plan_r2c_2d_many(hpxfft_plan *plan, size_t nx, size_t ny, size_t batch, size_t ld_in, size_t dist_in,
+                            size_t ld_out, size_t dist_out, int precision, int effort, void *stream)
+{
+    try
+    {
+        if (!plan) return fail(HPXFFT_INVALID_ARG, "null plan pointer");
+        *plan = nullptr;
+        if (precision < HPXFFT_F64 || precision > HPXFFT_BF16_STORAGE)
+            return fail(HPXFFT_INVALID_ARG, "unknown precision");
+        if (effort < HPXFFT_ESTIMATE || effort > HPXFFT_EXHAUSTIVE) return fail(HPXFFT_INVALID_ARG, "unknown effort");
+        if (batch == 0) return fail(HPXFFT_INVALID_ARG, "batch must be >= 1");
+        const auto t0 = std::chrono::steady_clock::now();
+        auto *p = new hpxfft_plan_s();
+        p->nx = nx;
+        p->ny = ny;
+        p->batch = batch;
+        p->ld_in = ld_in;
+        p->dist_in = batch > 1 ? dist_in : nx * ld_in;
+        p->ld_out = ld_out;
+        p->dist_out = batch > 1 ? dist_out : nx * ld_out;
+        p->prec = precision;
+        p->effort = effort;
+        p->stream = reinterpret_cast<hipStream_t>(stream);
+        if (int rc = plan_common(p))
+        {
+            std::string msg = g_last_error;
+            free_plan(p);
+            g_last_error = msg;
+            return rc;
+        }
+        p->plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *plan = p;
+        return HPXFFT_OK;
+    }
+    catch (const std::bad_alloc &)
+    {
+        return fail(HPXFFT_OUT_OF_MEMORY, "host allocation failed");
+    }
+    catch (const std::exception &e)
+    {
+        return fail(HPXFFT_HIP_ERROR, e.what());
+    }
+}
+
+int hpxfft_plan_r2c_2d(hpxfft_plan *plan, size_t nx, size_t ny, size_t ld_in, int precision, int effort, void *stream)
+{
+    const size_t ncy = ny / 2 + 1;
+    return hpxfft_plan_r2c_2d_many(plan, nx, ny, 1, ld_in, nx * ld_in, ncy, nx * ncy, precision, effort, stream);
+}
+
+int hpxfft_exec_r2c_2d(hpxfft_plan plan, const void *d_in, void *d_out)
+{
+    if (!plan || plan->batched) return fail(HPXFFT_INVALID_ARG, "invalid plan");
+    try
+    {
+        return exec_forward(plan, d_in, d_out, nullptr, nullptr);
+    }
+    catch (const std::exception &e)
+    {
+        return fail(HPXFFT_HIP_ERROR, e.what());
+    }
+}
+
+int hpxfft_exec_r2c_2d_scaled(hpxfft_plan plan, const void *d_in, void *d_out, const int *d_in_row_exp,
+                              int *d_out_col_exp)
+{
+    if (!plan || plan->batched) return fail(HPXFFT_INVALID_ARG, "invalid plan");
+    if (plan->prec != P_F16 && plan->prec != P_BF16)
+        return fail(HPXFFT_INVALID_ARG, "scaled exec is for 16-bit storage plans");
+    if (!d_out_col_exp) return fail(HPXFFT_INVALID_ARG, "d_out_col_exp is required");
+    try
+    {
+        return exec_forward(plan, d_in, d_out, d_in_row_exp, d_out_col_exp);
+    }
+    catch (const std::exception &e)
+    {
+        return fail(HPXFFT_HIP_ERROR, e.what());
+    }
+}
+
+int hpxfft_exec_r2c_2d_host(hpxfft_plan plan, double *inout)
+{
+    if (!plan || plan->batched) return fail(HPXFFT_INVALID_ARG, "invalid plan");
+    if (!inout) return fail(HPXFFT_INVALID_ARG, "null host pointer");
+    if (plan->prec != P_F64) return fail(HPXFFT_INVALID_ARG, "host exec is for F64 plans");
+    if (plan->ld_in != 2 * plan->ld_out || plan->batch != 1)
+        return fail(HPXFFT_INVALID_ARG, "host exec needs the in-place padded layout (ld_in == 2*(Ny/2+1))");
+    device_guard g(plan->device);
+    const size_t bytes = plan->nx * plan->ld_in * sizeof(double);
+    if (!plan->host_buf || plan->host_bytes < bytes)
+    {
+        (void)hipFree(plan->host_buf);
+        plan->host_buf = nullptr;
+        HIPCHK(hipMalloc(&plan->host_buf, bytes));
+        plan->host_bytes = bytes;
+    }
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, plan->stream));
+    HIPCHK(hipMemcpyAsync(plan->host_buf, inout, bytes, hipMemcpyHostToDevice, plan->stream));
+    if (int rc = exec_forward(plan, plan->host_buf, plan->host_buf, nullptr, nullptr)) return rc;
+    HIPCHK(hipMemcpyAsync(inout, plan->host_buf, bytes, hipMemcpyDeviceToHost, plan->stream));
+    HIPCHK(hipEventRecord(e1, plan->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    plan->h2d = elapsed(e0, plan->ev[0]);
+    plan->d2h = elapsed(plan->ev[2], e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return HPXFFT_OK;
+}
+
+int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_real)
+{
+    if (!plan || plan->batched) return fail(HPXFFT_INVALID_ARG, "invalid plan");
+    if (!d_spec || !d_real) return fail(HPXFFT_INVALID_ARG, "null device pointer");
+    if (plan->prec != P_F64 && plan->prec != P_F32)
+        return fail(HPXFFT_INVALID_ARG, "adjoint is implemented for F64 and F32 plans");
+    device_guard g(plan->device);
+    const size_t sb = storage_bytes(plan->prec);
+    const size_t bytes = plan->batch * plan->dist_out * 2 * sb;
+    if (!plan->adj_work)
+    {
+        HIPCHK(hipMalloc(&plan->adj_work, bytes));
+        plan->adj_bytes = bytes;
+    }
+    if (int rc = alloc_events(plan)) return rc;
+    HIPCHK(hipEventRecord(plan->ev[0], plan->stream));
+    if (int rc = run_columns(plan, d_spec, plan->adj_work, true, nullptr)) return rc;
+    HIPCHK(hipEventRecord(plan->ev[1], plan->stream));
+    RowArgs r{};
+    r.in = plan->adj_work;
+    r.out = d_real;
+    r.ld_in = (long long)(2 * plan->ld_out);
+    r.ld_out = (long long)plan->ld_in;
+    r.dist_in = (long long)(2 * plan->dist_out);
+    r.dist_out = (long long)plan->dist_in;
+    r.nrows = (int)plan->nx;
+    r.tw = tw_ptr(plan, plan->ny);
+    const geom rg = row_geom(plan->prec, plan->lgL);
+    const unsigned gx = (unsigned)((plan->nx + rg.nseq - 1) / rg.nseq);
+    HIPCHK(row_c2r(plan->prec, plan->lgL, r, gx, (unsigned)plan->batch, plan->stream));
+    HIPCHK(hipEventRecord(plan->ev[2], plan->stream));
+    plan->ran = true;
+    return HPXFFT_OK;
+}
+
+int hpxfft_get_measurement(hpxfft_plan plan, const char *key, double *value)
+{
+    if (!plan || !key || !value) return fail(HPXFFT_INVALID_ARG, "null argument");
+    const std::string k(key);
+    double v = 0.0;
+    if (k == "plan") v = plan->plan_seconds;
+    else if (k == "plan_flops") v = plan->plan_flops;
+    else if (k == "first_trans" || k == "second_trans") v = 0.0;
+    else if (k == "h2d") v = plan->h2d;
+    else if (k == "d2h") v = plan->d2h;
+    else if (k == "total" || k == "first_fftw" || k == "second_fftw")
+    {
+        if (plan->ran && !plan->batched)
+        {
+            device_guard g(plan->device);
+            HIPCHK(hipEventSynchronize(plan->ev[2]));
+            if (k == "total") v = elapsed(plan->ev[0], plan->ev[2]);
+            else if (k == "first_fftw") v = elapsed(plan->ev[0], plan->ev[1]);
+            else v = elapsed(plan->ev[1], plan->ev[2]);
+        }
+        else if (plan->ran && plan->batched)
+        {
+            device_guard g(plan->device);
+            HIPCHK(hipEventSynchronize(plan->ev[2]));
+            if (k == "total") v = elapsed(plan->ev[0], plan->ev[2]);
+        }
+    }
+    else
+    {
+        auto it = plan->extra.find(k);
+        if (it != plan->extra.end()) v = it->second;
+        else plan->extra[k] = 0.0;
+    }
+    *value = v;
+    return HPXFFT_OK;
+}
+
+int hpxfft_plan_describe(hpxfft_plan plan, char *buf, size_t len, size_t *needed)
+{
+    if (!plan) return fail(HPXFFT_INVALID_ARG, "null plan");
+    const std::string s = describe(plan);
+    if (needed) *needed = s.size();
+    if (buf && len)
+    {
+        const size_t n = std::min(len - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    return HPXFFT_OK;
+}
+
+int hpxfft_write_plan(hpxfft_plan plan, const char *path)
+{
+    if (!plan || !path) return fail(HPXFFT_INVALID_ARG, "null argument");
+    FILE *f = std::fopen(path, "a");
+    if (!f) return fail(HPXFFT_IO_ERROR, std::string("Failed to open file: ") + path);
+    const std::string s = describe(plan);
+    std::fprintf(f, "GPU r2c 2D plan:\n%s\n", s.c_str());
+    std::fclose(f);
+    return HPXFFT_OK;
+}
+
+int hpxfft_plan_destroy(hpxfft_plan plan)
+{
+    if (!plan) return HPXFFT_OK;
+    free_plan(plan);
+    return HPXFFT_OK;
+}
+
+int hpxfft_plan_r2c_2d_batched(hpxfft_plan *plan, size_t count, const size_t *nx, const size_t *ny, int precision,
+                               int effort, void *stream)
+{
+    (void)plan;
+    (void)count;
+    (void)nx;
+    (void)ny;
+    (void)precision;
+    (void)effort;
+    (void)stream;
+    return fail(HPXFFT_INVALID_ARG, "batched plans are not built yet");
+}
+
+int hpxfft_exec_r2c_2d_batched(hpxfft_plan plan, void *const *d_grids)
+{
+    (void)plan;
+    (void)d_grids;
+    return fail(HPXFFT_INVALID_ARG, "batched plans are not built yet");
+}
+
+}   // extern "C"

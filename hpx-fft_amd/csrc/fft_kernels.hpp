@@ -1,0 +1,497 @@
+// fft_kernels.hpp -- gfx950 kernels of the 2-D r2c path.
+//
+//   row_r2c_kernel : phase 1 of hpxfft::shared::loop::fft_2d_r2c_par
+//                    (core/src/shared/loop.cpp:61-69): 1-D r2c of length Ny
+//                    on every row, as an Ny/2-point complex Stockham FFT of
+//                    z[n] = x[2n] + i x[2n+1] plus the r2c split, in VGPRs/LDS.
+//   col_kernel     : phases 2-4 (loop.cpp:72-101): the transposes are fused
+//                    into the column kernel's row-segment loads/stores, and
+//                    the length-Nx c2c runs either in one tile (MODE 0) or as
+//                    a four-step Nx = P*Q split into MODE 1 (Q-point DFTs
+//                    over rows m1 + P*m2, twiddle w_Nx^{m1 k2}, to scratch)
+//                    and MODE 2 (P-point DFTs, to rows Q*k1 + k2).
+//   row_c2r_kernel : the row half of the adjoint (no reference counterpart):
+//                    halve interior bins, keep Re of DC/Nyquist, unnormalised
+//                    c2r of length Ny.
+#pragma once
+
+#include "fft_device.hpp"
+#include <type_traits>
+
+namespace hpxfft_dev {
+
+// ---------------------------------------------------------------------------
+// compile-time launch configurations (shared by host dispatch and kernels)
+template <typename T, typename S> struct cfg
+{
+    static constexpr int lds_budget = 64 * 1024;
+    // row kernel, L = 2^lg complex points per row
+    static constexpr int row_E(int lg) { return lg >= 4 ? 16 : (1 << lg); }
+    static constexpr int row_NT(int lg) { return lg <= 11 ? 128 : (1 << lg) / 16; }
+    static constexpr int row_NSEQ(int lg) { return row_NT(lg) * row_E(lg) / (1 << lg); }
+    static constexpr int lp(int L) { return L >= 16 ? L + L / 16 : L; }
+    static constexpr bool row_SPLIT(int lg)
+    {
+        return (long)row_NSEQ(lg) * lp(1 << lg) * 2 * (long)sizeof(T) > lds_budget;
+    }
+    // column tiles: W adjacent columns = one 128-byte row segment of storage
+    static constexpr int W = 128 / (2 * (int)sizeof(S)) > 32 ? 32 : 128 / (2 * (int)sizeof(S));
+    static constexpr int col_direct_max_lg = sizeof(S) == 2 ? 9 : 10;
+    static constexpr int col_E(int lg)
+    {
+        return (1 << lg) < 16 ? (1 << lg) : ((W << lg) <= 1024 && lg <= 8 ? 8 : 16);
+    }
+    static constexpr int col_NT(int lg) { return (W << lg) / col_E(lg); }
+    static constexpr bool col_SPLIT(int lg)
+    {
+        // LDS layout [e][W]: padded only when a W-row is narrower than 128 B
+        return (long)(W << lg) * 2 * (long)sizeof(T) > lds_budget;
+    }
+};
+
+struct RowArgs
+{
+    const void *in;
+    void *out;
+    long long ld_in;     // elements of S between input rows (reals)
+    long long ld_out;    // elements of S between output rows (reals; 2 per complex)
+    long long dist_in;   // between grids (batch)
+    long long dist_out;
+    int nrows;
+    const void *tw;        // w_{2L}^m, m < 2L  (compute precision)
+    const int *in_exp;     // optional per-row input exponent (value = stored * 2^-e)
+    int *out_exp;          // 16-bit outputs: per-row exponent written here
+    long long exp_dist;    // between grids, for in_exp/out_exp
+};
+
+struct ColArgs
+{
+    const void *in;    // spectrum (MODE 0,1) or scratch (MODE 2)
+    void *out;         // spectrum (MODE 0,2) or scratch (MODE 1)
+    long long pitch;   // complex elements between spectrum rows
+    long long dist;    // complex elements between spectrum grids (batch)
+    int ncols;         // valid columns of the spectrum (Ny/2+1)
+    int c0;            // first column of this launch (chunk start)
+    int CW;            // scratch chunk width (columns)
+    int P, Q;          // four-step split Nx = P*Q
+    const void *tw_L;  // w_L^m, m < L (sub-FFT length)
+    const void *tw_N;  // w_Nx^m, m < Nx (MODE 1 post-twiddle)
+    const int *row_exp;    // 16-bit inputs (MODE 0,1): per-row exponent of the rows
+    unsigned *colmax;      // MODE 1 writes (atomicMax of max |component| bits), MODE 2 reads
+    int *out_exp;          // 16-bit outputs (MODE 0,2): per-column exponent
+    long long rexp_dist;   // between grids for row_exp
+    long long cexp_dist;   // between grids for out_exp / colmax
+};
+
+template <typename S> constexpr bool is16() { return sizeof(S) == 2; }
+
+// exponent e with max * 2^e in [2^14, 2^15): keeps fp16 clear of overflow
+__device__ __forceinline__ int pick_exp(float m)
+{
+    if (!(m > 0.0f) || !isfinite(m)) return 0;
+    return 14 - ilogbf(m);
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, typename SI, typename SO, int LG>
+__global__ void __launch_bounds__((cfg<T, SI>::row_NT(LG)))
+row_r2c_kernel(RowArgs a)
+{
+    constexpr int L = 1 << LG;
+    using C = cfg<T, SI>;
+    constexpr int E = C::row_E(LG), NSEQ = C::row_NSEQ(LG);
+    constexpr bool SPLIT = C::row_SPLIT(LG);
+    using EN = Engine<T, L, E, NSEQ, false, SPLIT>;
+    constexpr int NT = EN::NT;
+    constexpr int HK = L / 2 + 1;                        // r2c pair tasks per row
+    constexpr int NTASK = (NSEQ * HK + NT - 1) / NT;
+
+    __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
+    __shared__ unsigned smax[is16<SO>() ? NSEQ : 1];
+
+    const int t = threadIdx.x;
+    const long long b = blockIdx.y;
+    const SI *in = reinterpret_cast<const SI *>(a.in) + b * a.dist_in;
+    SO *out = reinterpret_cast<SO *>(a.out) + b * a.dist_out;
+    const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(a.tw);
+    const int row0 = blockIdx.x * NSEQ;
+
+    cpx<T> v[E];
+#pragma unroll
+    for (int x = 0; x < E; ++x)
+    {
+        int seq, e;
+        EN::in_elem(t, x, seq, e);
+        const int row = row0 + seq;
+        cpx<T> z = {T(0), T(0)};
+        if (row < a.nrows)
+        {
+            z = stor<SI>::ld(in + (long long)row * a.ld_in + 2 * e);
+            if (a.in_exp) { const T s = (T)ldexp(1.0, -a.in_exp[b * a.exp_dist + row]); z = scal(z, s); }
+        }
+        v[x] = z;
+    }
+
+    EN::run(v, lds, tw, 2, t);
+
+    // r2c split: X[k] = Fe + w_{2L}^k Fo, X[L-k] = conj(Fe - w_{2L}^k Fo)
+    cpx<T> A[NTASK], B[NTASK];
+    if constexpr (!SPLIT)
+    {
+#pragma unroll
+        for (int x = 0; x < E; ++x) { int seq, e; EN::out_elem(t, x, seq, e); lds[EN::addr(seq, e)] = v[x]; }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NTASK; ++i)
+        {
+            const int tau = t + i * NT;
+            const int seq = tau / HK, k = tau - seq * HK;
+            if (tau < NSEQ * HK) { A[i] = lds[EN::addr(seq, k & (L - 1))]; B[i] = lds[EN::addr(seq, (L - k) & (L - 1))]; }
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int x = 0; x < E; ++x) { int seq, e; EN::out_elem(t, x, seq, e); lds[EN::addr(seq, e)] = v[x].x; }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NTASK; ++i)
+        {
+            const int tau = t + i * NT;
+            const int seq = tau / HK, k = tau - seq * HK;
+            if (tau < NSEQ * HK) { A[i].x = lds[EN::addr(seq, k & (L - 1))]; B[i].x = lds[EN::addr(seq, (L - k) & (L - 1))]; }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < E; ++x) { int seq, e; EN::out_elem(t, x, seq, e); lds[EN::addr(seq, e)] = v[x].y; }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NTASK; ++i)
+        {
+            const int tau = t + i * NT;
+            const int seq = tau / HK, k = tau - seq * HK;
+            if (tau < NSEQ * HK) { A[i].y = lds[EN::addr(seq, k & (L - 1))]; B[i].y = lds[EN::addr(seq, (L - k) & (L - 1))]; }
+        }
+    }
+
+    cpx<T> X0[NTASK], X1[NTASK];
+    float lmax[NTASK];
+#pragma unroll
+    for (int i = 0; i < NTASK; ++i)
+    {
+        const int tau = t + i * NT;
+        const int k = tau % HK;
+        const cpx<T> Ak = A[i], Bk = conjc(B[i]);
+        if (k == 0)
+        {
+            X0[i] = {Ak.x + Ak.y, T(0)};
+            X1[i] = {Ak.x - Ak.y, T(0)};
+        }
+        else
+        {
+            const cpx<T> Fe = scal(Ak + Bk, T(0.5));
+            const cpx<T> D = scal(Ak - Bk, T(0.5));
+            const cpx<T> Fo = {D.y, -D.x};
+            const cpx<T> w = tw[k];
+            const cpx<T> tt = cmul(w, Fo);
+            X0[i] = Fe + tt;
+            X1[i] = conjc(Fe - tt);
+        }
+        if constexpr (is16<SO>())
+        {
+            lmax[i] = (float)fmax(fmax(fabs(X0[i].x), fabs(X0[i].y)), fmax(fabs(X1[i].x), fabs(X1[i].y)));
+        }
+    }
+
+    if constexpr (is16<SO>())
+    {
+        for (int s = t; s < NSEQ; s += NT) smax[s] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NTASK; ++i)
+        {
+            const int tau = t + i * NT;
+            if (tau < NSEQ * HK) atomicMax(&smax[tau / HK], __float_as_uint(lmax[i]));
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < NTASK; ++i)
+    {
+        const int tau = t + i * NT;
+        const int seq = tau / HK, k = tau - seq * HK;
+        const int row = row0 + seq;
+        if (tau < NSEQ * HK && row < a.nrows)
+        {
+            cpx<T> x0 = X0[i], x1 = X1[i];
+            if constexpr (is16<SO>())
+            {
+                const int ex = pick_exp(__uint_as_float(smax[seq]));
+                const T s = (T)ldexpf(1.0f, ex);
+                x0 = scal(x0, s);
+                x1 = scal(x1, s);
+                if (k == 0 && a.out_exp) a.out_exp[b * a.exp_dist + row] = ex;
+            }
+            SO *orow = out + (long long)row * a.ld_out;
+            stor<SO>::st(orow + 2 * k, x0);
+            if (k == 0) stor<SO>::st(orow + 2 * L, x1);
+            else if (k != L - k) stor<SO>::st(orow + 2 * (L - k), x1);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// column tile kernel.  blockIdx.x = strip of W columns inside the launch's
+// chunk, blockIdx.y = m1 (MODE 1) / k2 (MODE 2) / 0, blockIdx.z = grid.
+template <typename T, typename SI, typename SO, typename SC, int LG, int MODE, bool CONJ>
+__global__ void __launch_bounds__((cfg<T, SC>::col_NT(LG)))
+col_kernel(ColArgs a)
+{
+    constexpr int L = 1 << LG;
+    using C = cfg<T, SC>;   // SC = spectrum storage type: fixes the tile width W
+    constexpr int W = C::W, E = C::col_E(LG);
+    constexpr bool SPLIT = C::col_SPLIT(LG);
+    using EN = Engine<T, L, E, W, true, SPLIT>;
+
+    __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
+    constexpr bool OUT16 = is16<SO>() && MODE != 1;
+    __shared__ unsigned smax[(OUT16 || (MODE == 1 && is16<SI>())) ? W : 1];
+
+    const int t = threadIdx.x;
+    const long long b = blockIdx.z;
+    const int sub = blockIdx.y;
+    const int col0 = a.c0 + blockIdx.x * W;
+    const int nvalid = min(W, a.ncols - col0);
+    const int lc0 = col0 - a.c0;   // column inside the scratch chunk
+
+    const SI *in = reinterpret_cast<const SI *>(a.in);
+    SO *out = reinterpret_cast<SO *>(a.out);
+    if constexpr (MODE != 2) in += b * a.dist;
+    if constexpr (MODE != 1) out += b * a.dist;
+
+    cpx<T> v[E];
+#pragma unroll
+    for (int x = 0; x < E; ++x)
+    {
+        int w, e;
+        EN::in_elem(t, x, w, e);
+        cpx<T> z = {T(0), T(0)};
+        if (w < nvalid)
+        {
+            long long idx;
+            int row = 0;
+            if constexpr (MODE == 0) { row = e; idx = (long long)row * a.pitch + col0 + w; }
+            else if constexpr (MODE == 1) { row = sub + a.P * e; idx = (long long)row * a.pitch + col0 + w; }
+            else { idx = ((long long)sub * a.P + e) * a.CW + lc0 + w; }
+            z = stor<SI>::ld(in + 2 * idx);
+            if constexpr (is16<SI>() && MODE != 2)
+            {
+                if (a.row_exp) z = scal(z, (T)ldexpf(1.0f, -a.row_exp[b * a.rexp_dist + row]));
+            }
+            if constexpr (CONJ && MODE != 2) z = conjc(z);
+        }
+        v[x] = z;
+    }
+
+    EN::run(v, lds, reinterpret_cast<const cpx<T> *>(a.tw_L), 1, t);
+
+    if constexpr (MODE == 1)
+    {
+        const cpx<T> *twN = reinterpret_cast<const cpx<T> *>(a.tw_N);
+#pragma unroll
+        for (int x = 0; x < E; ++x)
+        {
+            int w, e;
+            EN::out_elem(t, x, w, e);
+            if (sub != 0) v[x] = cmul(v[x], twN[(long long)sub * e]);
+        }
+    }
+
+    // per-column maxima (16-bit outputs need a per-column exponent)
+    constexpr bool CA16 = MODE == 1 && is16<SI>();
+    if constexpr (CA16 || OUT16)
+    {
+        for (int s = t; s < W; s += EN::NT) smax[s] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < E; ++x)
+        {
+            int w, e;
+            EN::out_elem(t, x, w, e);
+            const float m = (float)fmax(fabs(v[x].x), fabs(v[x].y));
+            atomicMax(&smax[w], __float_as_uint(m));
+        }
+        __syncthreads();
+        if constexpr (CA16)
+        {
+            if (a.colmax && t < nvalid) atomicMax(&a.colmax[b * a.cexp_dist + col0 + t], smax[t]);
+        }
+    }
+
+#pragma unroll
+    for (int x = 0; x < E; ++x)
+    {
+        int w, e;
+        EN::out_elem(t, x, w, e);
+        if (w < nvalid)
+        {
+            cpx<T> z = v[x];
+            long long idx;
+            if constexpr (MODE == 0) idx = (long long)e * a.pitch + col0 + w;
+            else if constexpr (MODE == 1) idx = ((long long)e * a.P + sub) * a.CW + lc0 + w;
+            else idx = ((long long)a.Q * e + sub) * a.pitch + col0 + w;
+            if constexpr (CONJ && MODE != 1) z = conjc(z);
+            if constexpr (OUT16)
+            {
+                int ex;
+                if constexpr (MODE == 0) ex = pick_exp(__uint_as_float(smax[w]));
+                else
+                {
+                    // bound |X[Q k1 + k2]| <= sum_m1 |Z[m1][k2]| <= P * sqrt(2) * max component of Z
+                    const float cm = __uint_as_float(a.colmax[b * a.cexp_dist + col0 + w]);
+                    ex = pick_exp(cm * 1.5f * (float)a.P);
+                }
+                z = scal(z, (T)ldexpf(1.0f, ex));
+                if (e == 0 && a.out_exp) a.out_exp[b * a.cexp_dist + col0 + w] = ex;
+            }
+            stor<SO>::st(out + 2 * idx, z);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row c2r for the adjoint.  Input: half-spectrum rows G[0..L] (complex, SI),
+// output: real rows (SO).  out[2n] + i out[2n+1] = IDFT_L(Z),
+//   H = G/2 on 0<k<L, H[0] = Re G[0], H[L] = Re G[L],
+//   Z[k] = (H[k] + conj H[L-k]) + i (H[k] - conj H[L-k]) conj(w_{2L}^k).
+// The inverse DFT is computed as conj(DFT(conj Z)).
+template <typename T, typename SI, typename SO, int LG>
+__global__ void __launch_bounds__((cfg<T, SI>::row_NT(LG)))
+row_c2r_kernel(RowArgs a)
+{
+    constexpr int L = 1 << LG;
+    using C = cfg<T, SI>;
+    constexpr int E = C::row_E(LG), NSEQ = C::row_NSEQ(LG);
+    constexpr bool SPLIT = C::row_SPLIT(LG);
+    using EN = Engine<T, L, E, NSEQ, false, SPLIT>;
+    constexpr int NT = EN::NT;
+    constexpr int HK = L / 2 + 1;
+    constexpr int NTASK = (NSEQ * HK + NT - 1) / NT;
+
+    __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
+
+    const int t = threadIdx.x;
+    const long long b = blockIdx.y;
+    const SI *in = reinterpret_cast<const SI *>(a.in) + b * a.dist_in;
+    SO *out = reinterpret_cast<SO *>(a.out) + b * a.dist_out;
+    const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(a.tw);
+    const int row0 = blockIdx.x * NSEQ;
+
+    // pre-processing: pairs (k, L-k) -> conj Z[k], conj Z[L-k]
+    cpx<T> Z0[NTASK], Z1[NTASK];
+#pragma unroll
+    for (int i = 0; i < NTASK; ++i)
+    {
+        const int tau = t + i * NT;
+        const int seq = tau / HK, k = tau - seq * HK;
+        const int row = row0 + seq;
+        cpx<T> g0 = {T(0), T(0)}, g1 = {T(0), T(0)};
+        if (tau < NSEQ * HK && row < a.nrows)
+        {
+            const SI *irow = in + (long long)row * a.ld_in;
+            g0 = stor<SI>::ld(irow + 2 * k);
+            g1 = stor<SI>::ld(irow + 2 * (L - k));
+        }
+        if (k == 0)
+        {
+            // H0 = Re G0, HL = Re GL: Z0 = (H0 + HL) + i (H0 - HL)
+            Z0[i] = conjc(cpx<T>{g0.x + g1.x, g0.x - g1.x});
+            Z1[i] = Z0[i];
+        }
+        else
+        {
+            const cpx<T> h0 = scal(g0, T(0.5)), h1 = scal(g1, T(0.5));
+            const cpx<T> w = tw[k];
+            const cpx<T> wc = conjc(w);
+            // Z[k]
+            const cpx<T> s0 = h0 + conjc(h1), d0 = cmul(h0 - conjc(h1), wc);
+            const cpx<T> zk = {s0.x - d0.y, s0.y + d0.x};   // s0 + i d0
+            // Z[L-k]: conj(w_{2L}^{L-k}) = -w_{2L}^k
+            const cpx<T> s1 = h1 + conjc(h0), d1 = cmul(h1 - conjc(h0), cpx<T>{-w.x, -w.y});
+            const cpx<T> zlk = {s1.x - d1.y, s1.y + d1.x};
+            Z0[i] = conjc(zk);
+            Z1[i] = conjc(zlk);
+        }
+    }
+    cpx<T> v[E];
+    if constexpr (!SPLIT)
+    {
+#pragma unroll
+        for (int i = 0; i < NTASK; ++i)
+        {
+            const int tau = t + i * NT;
+            const int seq = tau / HK, k = tau - seq * HK;
+            if (tau < NSEQ * HK)
+            {
+                lds[EN::addr(seq, k & (L - 1))] = Z0[i];
+                if (k != 0) lds[EN::addr(seq, L - k)] = Z1[i];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int x = 0; x < E; ++x) { int seq, e; EN::in_elem(t, x, seq, e); v[x] = lds[EN::addr(seq, e)]; }
+        __syncthreads();
+    }
+    else
+    {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+        {
+#pragma unroll
+            for (int i = 0; i < NTASK; ++i)
+            {
+                const int tau = t + i * NT;
+                const int seq = tau / HK, k = tau - seq * HK;
+                if (tau < NSEQ * HK)
+                {
+                    lds[EN::addr(seq, k & (L - 1))] = h ? Z0[i].y : Z0[i].x;
+                    if (k != 0) lds[EN::addr(seq, L - k)] = h ? Z1[i].y : Z1[i].x;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int x = 0; x < E; ++x)
+            {
+                int seq, e;
+                EN::in_elem(t, x, seq, e);
+                if (h) v[x].y = lds[EN::addr(seq, e)];
+                else v[x].x = lds[EN::addr(seq, e)];
+            }
+            __syncthreads();
+        }
+    }
+
+    EN::run(v, lds, tw, 2, t);
+
+#pragma unroll
+    for (int x = 0; x < E; ++x)
+    {
+        int seq, e;
+        EN::out_elem(t, x, seq, e);
+        const int row = row0 + seq;
+        if (row < a.nrows)
+        {
+            // u = DFT(conj Z); z = conj(u): out[2e] = Re u, out[2e+1] = -Im u
+            stor<SO>::st(out + (long long)row * a.ld_out + 2 * e, cpx<T>{v[x].x, -v[x].y});
+        }
+    }
+}
+
+template <int UNUSED = 0> __global__ void zero_u32_kernel(unsigned *p, long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+}   // namespace hpxfft_dev

@@ -1,0 +1,217 @@
+// launch.hpp -- host-side dispatch from runtime sizes to kernel instantiations.
+#pragma once
+
+#include "fft_kernels.hpp"
+
+namespace hpxfft_host {
+
+using hpxfft_dev::ColArgs;
+using hpxfft_dev::RowArgs;
+
+enum prec_id { P_F64 = 0, P_F32 = 1, P_F16 = 2, P_BF16 = 3 };
+
+constexpr int ROW_MAX_LG = 14;       // Ny <= 32768
+constexpr int COL_TILE_MIN_LG = 5;   // four-step sub-FFT lengths 32..256
+constexpr int COL_TILE_MAX_LG = 8;
+
+struct geom
+{
+    int nt;     // threads per workgroup
+    int nseq;   // rows (row kernels) or columns W (column kernels) per workgroup
+    int E;      // elements per thread
+    bool split; // LDS exchange by real/imag halves
+};
+
+// per precision (defined in inst_<prec>.hip)
+#define HPXFFT_DECL_LAUNCH(sfx)                                                                                    \
+    hipError_t row_r2c_##sfx(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s);                   \
+    hipError_t row_c2r_##sfx(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s);                   \
+    hipError_t col_##sfx(int lg, int mode, bool conj, const ColArgs &a, unsigned gx, unsigned gy, unsigned gz,    \
+                         hipStream_t s);                                                                           \
+    geom row_geom_##sfx(int lg);                                                                                   \
+    geom col_geom_##sfx(int lg);                                                                                   \
+    int col_direct_max_lg_##sfx();
+
+HPXFFT_DECL_LAUNCH(f64)
+HPXFFT_DECL_LAUNCH(f32)
+HPXFFT_DECL_LAUNCH(f16)
+HPXFFT_DECL_LAUNCH(bf16)
+
+inline hipError_t row_r2c(int p, int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+{
+    switch (p)
+    {
+    case P_F64: return row_r2c_f64(lg, a, gx, gy, s);
+    case P_F32: return row_r2c_f32(lg, a, gx, gy, s);
+    case P_F16: return row_r2c_f16(lg, a, gx, gy, s);
+    default: return row_r2c_bf16(lg, a, gx, gy, s);
+    }
+}
+inline hipError_t row_c2r(int p, int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+{
+    switch (p)
+    {
+    case P_F64: return row_c2r_f64(lg, a, gx, gy, s);
+    case P_F32: return row_c2r_f32(lg, a, gx, gy, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+inline hipError_t col(int p, int lg, int mode, bool conj, const ColArgs &a, unsigned gx, unsigned gy, unsigned gz,
+                      hipStream_t s)
+{
+    switch (p)
+    {
+    case P_F64: return col_f64(lg, mode, conj, a, gx, gy, gz, s);
+    case P_F32: return col_f32(lg, mode, conj, a, gx, gy, gz, s);
+    case P_F16: return col_f16(lg, mode, conj, a, gx, gy, gz, s);
+    default: return col_bf16(lg, mode, conj, a, gx, gy, gz, s);
+    }
+}
+inline geom row_geom(int p, int lg)
+{
+    switch (p)
+    {
+    case P_F64: return row_geom_f64(lg);
+    case P_F32: return row_geom_f32(lg);
+    case P_F16: return row_geom_f16(lg);
+    default: return row_geom_bf16(lg);
+    }
+}
+inline geom col_geom(int p, int lg)
+{
+    switch (p)
+    {
+    case P_F64: return col_geom_f64(lg);
+    case P_F32: return col_geom_f32(lg);
+    case P_F16: return col_geom_f16(lg);
+    default: return col_geom_bf16(lg);
+    }
+}
+inline int col_direct_max_lg(int p)
+{
+    switch (p)
+    {
+    case P_F64: return col_direct_max_lg_f64();
+    case P_F32: return col_direct_max_lg_f32();
+    case P_F16: return col_direct_max_lg_f16();
+    default: return col_direct_max_lg_bf16();
+    }
+}
+
+}   // namespace hpxfft_host
+
+// ---------------------------------------------------------------------------
+// instantiation helpers, used by inst_<prec>.hip only
+#ifdef HPXFFT_INSTANTIATE
+namespace hpxfft_host {
+namespace detail {
+using namespace hpxfft_dev;
+
+template <typename T, typename S, int LG>
+hipError_t launch_row_r2c(const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+{
+    hipLaunchKernelGGL((row_r2c_kernel<T, S, S, LG>), dim3(gx, gy), dim3(cfg<T, S>::row_NT(LG)), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T, typename S, int LG>
+hipError_t launch_row_c2r(const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+{
+    hipLaunchKernelGGL((row_c2r_kernel<T, S, S, LG>), dim3(gx, gy), dim3(cfg<T, S>::row_NT(LG)), 0, s, a);
+    return hipGetLastError();
+}
+template <typename T, typename S, int LG, int MODE, bool CONJ>
+hipError_t launch_col(const ColArgs &a, unsigned gx, unsigned gy, unsigned gz, hipStream_t s)
+{
+    // MODE 1 writes the compute-precision scratch, MODE 2 reads it
+    using SI = typename std::conditional<MODE == 2, T, S>::type;
+    using SO = typename std::conditional<MODE == 1, T, S>::type;
+    hipLaunchKernelGGL((col_kernel<T, SI, SO, S, LG, MODE, CONJ>), dim3(gx, gy, gz), dim3(cfg<T, S>::col_NT(LG)), 0,
+                       s, a);
+    return hipGetLastError();
+}
+
+template <typename T, typename S, int LG = 0>
+hipError_t row_r2c_sw(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+{
+    if constexpr (LG > ROW_MAX_LG) return hipErrorInvalidValue;
+    else return lg == LG ? launch_row_r2c<T, S, LG>(a, gx, gy, s) : row_r2c_sw<T, S, LG + 1>(lg, a, gx, gy, s);
+}
+template <typename T, typename S, int LG = 0>
+hipError_t row_c2r_sw(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+{
+    if constexpr (LG > ROW_MAX_LG) return hipErrorInvalidValue;
+    else return lg == LG ? launch_row_c2r<T, S, LG>(a, gx, gy, s) : row_c2r_sw<T, S, LG + 1>(lg, a, gx, gy, s);
+}
+template <typename T, typename S, bool ADJ, int LG = 0>
+hipError_t col_sw(int lg, int mode, bool conj, const ColArgs &a, unsigned gx, unsigned gy, unsigned gz, hipStream_t s)
+{
+    constexpr int DMAX = cfg<T, S>::col_direct_max_lg;
+    if constexpr (LG > DMAX && LG > COL_TILE_MAX_LG) return hipErrorInvalidValue;
+    else
+    {
+        if (lg == LG)
+        {
+            if (mode == 0)
+            {
+                if constexpr (LG <= DMAX)
+                {
+                    if (!conj) return launch_col<T, S, LG, 0, false>(a, gx, gy, gz, s);
+                    if constexpr (ADJ) return launch_col<T, S, LG, 0, true>(a, gx, gy, gz, s);
+                }
+                return hipErrorInvalidValue;
+            }
+            if constexpr (LG >= COL_TILE_MIN_LG && LG <= COL_TILE_MAX_LG)
+            {
+                if (!conj)
+                    return mode == 1 ? launch_col<T, S, LG, 1, false>(a, gx, gy, gz, s)
+                                     : launch_col<T, S, LG, 2, false>(a, gx, gy, gz, s);
+                if constexpr (ADJ)
+                    return mode == 1 ? launch_col<T, S, LG, 1, true>(a, gx, gy, gz, s)
+                                     : launch_col<T, S, LG, 2, true>(a, gx, gy, gz, s);
+            }
+            return hipErrorInvalidValue;
+        }
+        return col_sw<T, S, ADJ, LG + 1>(lg, mode, conj, a, gx, gy, gz, s);
+    }
+}
+
+template <typename T, typename S, bool ADJ>
+hipError_t row_c2r_maybe(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+{
+    if constexpr (ADJ) return row_c2r_sw<T, S>(lg, a, gx, gy, s);
+    else return hipErrorInvalidValue;
+}
+
+template <typename T, typename S> geom row_geom_t(int lg)
+{
+    using C = cfg<T, S>;
+    return geom{C::row_NT(lg), C::row_NSEQ(lg), C::row_E(lg), C::row_SPLIT(lg)};
+}
+template <typename T, typename S> geom col_geom_t(int lg)
+{
+    using C = cfg<T, S>;
+    return geom{C::col_NT(lg), C::W, C::col_E(lg), C::col_SPLIT(lg)};
+}
+}   // namespace detail
+}   // namespace hpxfft_host
+
+#define HPXFFT_DEFINE_LAUNCH(sfx, T, S, ADJ)                                                                         \
+    namespace hpxfft_host {                                                                                        \
+    hipError_t row_r2c_##sfx(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)                    \
+    {                                                                                                              \
+        return detail::row_r2c_sw<T, S>(lg, a, gx, gy, s);                                                         \
+    }                                                                                                              \
+    hipError_t row_c2r_##sfx(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)                    \
+    {                                                                                                              \
+        return detail::row_c2r_maybe<T, S, ADJ>(lg, a, gx, gy, s);                                                 \
+    }                                                                                                              \
+    hipError_t col_##sfx(int lg, int mode, bool conj, const ColArgs &a, unsigned gx, unsigned gy, unsigned gz,     \
+                         hipStream_t s)                                                                            \
+    {                                                                                                              \
+        return detail::col_sw<T, S, ADJ>(lg, mode, conj, a, gx, gy, gz, s);                                        \
+    }                                                                                                              \
+    geom row_geom_##sfx(int lg) { return detail::row_geom_t<T, S>(lg); }                                           \
+    geom col_geom_##sfx(int lg) { return detail::col_geom_t<T, S>(lg); }                                           \
+    int col_direct_max_lg_##sfx() { return hpxfft_dev::cfg<T, S>::col_direct_max_lg; }                             \
+    }
+#endif

@@ -1,0 +1,153 @@
+/*
+ * hpxfft_hip.h -- C ABI of the MI355X (gfx950) 2-D real-to-complex FFT.
+ *
+ * Plain C: pointers, sizes and status codes; no C++ or torch types and no
+ * exception ever crosses this boundary.  Every entry point below replaces a
+ * member of HPX-FFT's shared-memory class surface (hpxfft::shared::loop,
+ * core/include/hpxfft/shared/loop.hpp:15-57) or of its FFTW adapter
+ * (core/include/hpxfft/util/adapter_fftw.hpp:9-79); the citation is on each
+ * declaration.  The C++ drop-in class include/hpxfft/gpu/loop.hpp is a thin
+ * header-only layer over these functions.
+ *
+ * Transform (FFTW_FORWARD sign, unnormalised, as FFTW's r2c used by the
+ * reference, core/src/util/adapter_fftw.cpp:9,28-29):
+ *     X[kx][ky] = sum_m sum_n x[m][n] exp(-2 pi i (kx m / Nx + ky n / Ny)),
+ *     kx in [0,Nx), ky in [0, Ny/2].
+ * Layouts: input row m of grid g at in[g*dist_in + m*ld_in + n] (reals);
+ * output X[kx][ky] at out[g*dist_out + kx*ld_out + ky] (complex = 2 reals,
+ * interleaved re,im).  In place (out == in) needs ld_in == 2*ld_out and
+ * dist_in == 2*dist_out (the reference's padded vector_2d(Nx, 2(Ny/2+1))).
+ *
+ * Threading: a plan is bound to one HIP stream; calls on one plan must be
+ * serialised by the caller; distinct plans are independent.
+ */
+#ifndef HPXFFT_HIP_H_INCLUDED
+#define HPXFFT_HIP_H_INCLUDED
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference reports errors only by exceptions
+ * (std::invalid_argument, adapter_fftw.hpp:42; std::runtime_error,
+ * loop.cpp:200); the C++ layer maps these codes back to those types. */
+typedef enum
+{
+    HPXFFT_OK = 0,
+    HPXFFT_INVALID_ARG = 1,      /* bad pointer, flag, layout, or call order */
+    HPXFFT_UNSUPPORTED_SIZE = 2, /* Nx or Ny not a power of two, or out of range */
+    HPXFFT_NO_DEVICE = 3,        /* no HIP device visible */
+    HPXFFT_OUT_OF_MEMORY = 4,
+    HPXFFT_HIP_ERROR = 5,
+    HPXFFT_IO_ERROR = 6
+} hpxfft_status;
+
+/* Arithmetic / storage of a plan. */
+typedef enum
+{
+    HPXFFT_F64 = 0,          /* FP64 storage and arithmetic (the reference's `real` = double, loop.hpp:9) */
+    HPXFFT_F32 = 1,          /* FP32 storage and arithmetic, twiddles generated in FP64 then rounded */
+    HPXFFT_F16_STORAGE = 2,  /* IEEE half storage, FP32 arithmetic, power-of-two scale per row of each pass */
+    HPXFFT_BF16_STORAGE = 3  /* bfloat16 storage, FP32 arithmetic, same scaling */
+} hpxfft_precision;
+
+/* Planning effort, the image of the FFTW plan-flag strings. */
+typedef enum
+{
+    HPXFFT_ESTIMATE = 0,
+    HPXFFT_MEASURE = 1,    /* time the candidate kernel configurations and keep the fastest */
+    HPXFFT_PATIENT = 2,
+    HPXFFT_EXHAUSTIVE = 3
+} hpxfft_effort;
+
+typedef struct hpxfft_plan_s *hpxfft_plan;
+
+/* Replaces hpxfft::util::fftw_adapter::string_to_fftw_plan_flag
+ * (core/include/hpxfft/util/adapter_fftw.hpp:22-44): "estimate", "measure",
+ * "patient", "exhaustive" -> effort; anything else -> HPXFFT_INVALID_ARG with
+ * last_error "Invalid FFTW plan flag string". */
+int hpxfft_plan_flag_from_string(const char *flag, int *effort);
+
+/* Replaces hpxfft::shared::loop::initialize (core/src/shared/loop.cpp:158-189)
+ * for one in-place grid: Nx rows, Ny reals per row, row pitch ld_in reals
+ * (the reference derives Nx = n_row, Ny = n_col - 2 from the vector_2d,
+ * loop.cpp:163-165).  Output pitch is Ny/2+1 complex; exec may be in place
+ * when ld_in == 2*(Ny/2+1).  `stream` is a hipStream_t (NULL = default). */
+int hpxfft_plan_r2c_2d(hpxfft_plan *plan, size_t nx, size_t ny, size_t ld_in, int precision, int effort,
+                       void *stream);
+
+/* Same as hpxfft_plan_r2c_2d for `batch` grids with explicit layouts (no
+ * reference counterpart for the batch; used by the torch binding). */
+int hpxfft_plan_r2c_2d_many(hpxfft_plan *plan, size_t nx, size_t ny, size_t batch, size_t ld_in, size_t dist_in,
+                            size_t ld_out, size_t dist_out, int precision, int effort, void *stream);
+
+/* Replaces the body of hpxfft::shared::loop::fft_2d_r2c_par / _seq
+ * (core/src/shared/loop.cpp:56-155): row r2c, transpose, column c2c,
+ * transpose back, on device memory, enqueued on the plan's stream (no host
+ * synchronisation).  d_in/d_out are device pointers of the plan's storage
+ * type (double, float, _Float16 or bfloat16 bit patterns). */
+int hpxfft_exec_r2c_2d(hpxfft_plan plan, const void *d_in, void *d_out);
+
+/* 16-bit storage plans: value = stored * 2^-exp.  d_in_row_exp (optional,
+ * Nx ints per grid) scales the input rows; d_out_col_exp (required, Ny/2+1
+ * ints per grid) receives the per-column exponent of the output (the second
+ * pass works on columns, i.e. on the rows of the reference's transposed
+ * buffer, loop.cpp:167). */
+int hpxfft_exec_r2c_2d_scaled(hpxfft_plan plan, const void *d_in, void *d_out, const int *d_in_row_exp,
+                              int *d_out_col_exp);
+
+/* Drop-in host path for fft_2d_r2c_par (loop.cpp:56-113) on the reference's
+ * in-place padded buffer: copies `inout` (Nx x ld_in doubles) to the device,
+ * transforms, copies back and synchronises.  F64 plans only.  Adds the
+ * measurement keys "h2d" and "d2h". */
+int hpxfft_exec_r2c_2d_host(hpxfft_plan plan, double *inout);
+
+/* Adjoint of the r2c map under Re<A,B> on the half spectrum (no reference
+ * counterpart):  (R* Y)[m][n] = Re sum_kx sum_{ky<=Ny/2} Y[kx][ky]
+ * exp(+2 pi i (kx m/Nx + ky n/Ny)).  d_spec uses the plan's output layout
+ * and is not modified; d_real uses the plan's input layout.  F64/F32 plans. */
+int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_real);
+
+/* Replaces hpxfft::shared::loop::get_measurement (loop.cpp:192): seconds of
+ * the last exec for "total", "first_fftw", "first_trans", "second_fftw",
+ * "second_trans" (the transposes are fused into the column pass and read
+ * 0), "h2d", "d2h"; "plan" (planning seconds) and "plan_flops".  Unknown keys
+ * give 0.0, as std::map::operator[] does in the reference.  Waits for the
+ * last exec to finish. */
+int hpxfft_get_measurement(hpxfft_plan plan, const char *key, double *value);
+
+/* Replaces hpxfft::shared::loop::write_plans_to_file (loop.cpp:194-212):
+ * appends a description of the chosen kernels to `path`; HPXFFT_IO_ERROR
+ * with last_error "Failed to open file: <path>" if it cannot be opened. */
+int hpxfft_write_plan(hpxfft_plan plan, const char *path);
+
+/* Plan text (same content as hpxfft_write_plan) into buf (NUL-terminated,
+ * truncated to len).  Returns the full length in *needed if non-NULL. */
+int hpxfft_plan_describe(hpxfft_plan plan, char *buf, size_t len, size_t *needed);
+
+/* Replaces ~loop (loop.hpp:30): frees device buffers and the stream state. */
+int hpxfft_plan_destroy(hpxfft_plan plan);
+
+/* Batched uneven grids (no reference counterpart): `count` independent
+ * in-place grids, grid i of nx[i] x ny[i] in the padded layout
+ * (ld = 2*(ny[i]/2+1) reals), all transformed by one persistent launch.
+ * F32 only. */
+int hpxfft_plan_r2c_2d_batched(hpxfft_plan *plan, size_t count, const size_t *nx, const size_t *ny, int precision,
+                               int effort, void *stream);
+
+/* d_grids: host array of `count` device pointers, one per grid (in place). */
+int hpxfft_exec_r2c_2d_batched(hpxfft_plan plan, void *const *d_grids);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *hpxfft_last_error(void);
+
+/* Library version string. */
+const char *hpxfft_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HPXFFT_HIP_H_INCLUDED */
