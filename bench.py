@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark: 2-D r2c FFT throughput on one MI355X (gfx950).
+
+Workload (the headline configuration): Nx = Ny = 2^14 real input, uniform
+[-1,1) synthetic data, FP64 storage and arithmetic, forward transform, data
+resident in HBM.  One step = one full 2-D r2c (row pass + column pass) of the
+grid.  Also measured: the same grid in FP32 (speed-up and rel-L2 vs FP64).
+
+value = algorithmic bytes / time, 2-pass model (transposes not credited):
+    B = [Nx*Ny*s + H] + [2H],  H = Nx*(Ny/2+1)*2*s,  s = bytes per real.
+
+Prints ONE JSON line.  Flags: --steps K --warmup W --gpus N (N must be 1)
+--no-cpu (skip the CPU baseline leg) --nx/--ny (other sizes).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for _p in (REPO, os.path.join(REPO, "hpx-fft_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, published
+FP64_VEC_TFLOPS = 78.6         # MI355X, published
+FP32_VEC_TFLOPS = 157.3
+
+
+def alg_bytes(nx: int, ny: int, s: int) -> dict:
+    H = nx * (ny // 2 + 1) * 2 * s
+    row = nx * ny * s + H
+    col = 2 * H
+    return {"row": row, "col": col, "total": row + col}
+
+
+def nominal_flops(nx: int, ny: int) -> float:
+    n = nx * ny
+    return 5.0 * n * math.log2(n)
+
+
+def time_gpu(nx: int, ny: int, precision: str, steps: int, warmup: int, seed: int):
+    import torch
+    import hpxfft_amd
+
+    dt = torch.float64 if precision == "f64" else torch.float32
+    ld = 2 * (ny // 2 + 1)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    x = torch.empty((nx, ld), dtype=dt, device="cuda")
+    x[:, :ny].uniform_(-1.0, 1.0, generator=g)
+    x[:, ny:].zero_()
+    out = torch.empty((nx, ny // 2 + 1), dtype=torch.complex128 if precision == "f64" else torch.complex64,
+                      device="cuda")
+    stream = torch.cuda.current_stream()
+    plan = hpxfft_amd.Plan(nx, ny, ld_in=ld, precision=precision, stream=stream)
+    for _ in range(warmup):
+        plan.exec(x.data_ptr(), out.data_ptr())
+    torch.cuda.synchronize()
+    # timed region: K steps back to back on the plan's stream
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        plan.exec(x.data_ptr(), out.data_ptr())
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / steps
+    # per-phase times from the plan's own HIP events (row pass, column pass)
+    rows, cols = [], []
+    for _ in range(max(3, min(steps, 10))):
+        plan.exec(x.data_ptr(), out.data_ptr())
+        rows.append(plan.measurement("first_fftw"))
+        cols.append(plan.measurement("second_fftw"))
+    desc = plan.describe()
+    plan.close()
+    return {"ms_per_step": ms, "wall_s": wall, "row_s": sorted(rows)[len(rows) // 2],
+            "col_s": sorted(cols)[len(cols) // 2], "out": out, "x": x, "plan": desc}
+
+
+def cpu_baseline(nx: int, ny: int, budget_s: float = 20.0):
+    """The oracle's restatement of shared::loop::fft_2d_r2c_par on host cores."""
+    import oracle
+
+    threads = min(16, len(os.sched_getaffinity(0)))
+    x = oracle.fill(nx, ny, seed=2, dist="upm1")
+    loop = oracle.Loop(nx, ny)
+    t0 = time.perf_counter()
+    loop.exec(x, nthreads=threads)   # warm-up (also first touch of the transpose buffer)
+    first = time.perf_counter() - t0
+    reps = max(1, min(3, int(budget_s / max(first, 1e-3)) - 1))
+    ts = []
+    for _ in range(reps):
+        x = oracle.fill(nx, ny, seed=2, dist="upm1")
+        loop.exec(x, nthreads=threads)
+        ts.append(loop.measurements["total"])
+    t = sorted(ts)[len(ts) // 2]
+    B = alg_bytes(nx, ny, 8)["total"]
+    return {"value": B / t / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": f"full {nx}x{ny} FP64 grid, oracle restatement of shared::loop::fft_2d_r2c_par "
+                      f"(OpenMP, radix-2 FFT), median of {reps} after 1 warm-up",
+            "seconds": t, "phases": dict(loop.measurements)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--nx", type=int, default=1 << 14)
+    ap.add_argument("--ny", type=int, default=1 << 14)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-f32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the 2-D r2c path runs on a single MI355X (no multi-GPU path is built)",
+              file=sys.stderr)
+        return 2
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 2
+    nx, ny = args.nx, args.ny
+    r64 = time_gpu(nx, ny, "f64", args.steps, args.warmup, seed=2)
+    b64 = alg_bytes(nx, ny, 8)
+    gbs = b64["total"] / (r64["ms_per_step"] * 1e-3) / 1e9
+    # dominant kernel phase: row pass or column pass (whichever is longer)
+    phases = {"row_r2c": (b64["row"], r64["row_s"]), "column_c2c": (b64["col"], r64["col_s"])}
+    dom = max(phases, key=lambda k: phases[k][1])
+    dom_bytes, dom_s = phases[dom]
+    ach = dom_bytes / dom_s / 1e9
+    result = {
+        "metric": "2D r2c FFT GB/s & %HBM peak at 2^14^2, FP32 vs FP64 at stated L2 error",
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(r64["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: uniform [-1,1) from a seeded generator, resident in HBM",
+        "config": {"workload": f"2-D r2c FFT {nx}x{ny} FP64 (out of place, padded input layout)",
+                   "nx": nx, "ny": ny, "parallelism": "single GPU"},
+        "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+        "gflops": round(nominal_flops(nx, ny) / (r64["ms_per_step"] * 1e-3) / 1e9, 1),
+        "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "alg_bytes_per_launch": dom_bytes, "avg_launch_s": dom_s},
+        "phases_s": {"row_r2c": r64["row_s"], "column_c2c": r64["col_s"]},
+    }
+    result["fp64_vector_frac"] = round(result["gflops"] / (FP64_VEC_TFLOPS * 1e3), 4)
+    if not args.no_f32:
+        r32 = time_gpu(nx, ny, "f32", args.steps, args.warmup, seed=2)
+        b32 = alg_bytes(nx, ny, 4)
+        # FP32 error against the FP64 result of the same input (the FP64 path matches the oracle to 1e-12)
+        x64 = r64["x"]
+        import hpxfft_amd
+        x32 = x64.float()
+        out32 = torch.empty((nx, ny // 2 + 1), dtype=torch.complex64, device="cuda")
+        p32 = hpxfft_amd.Plan(nx, ny, ld_in=2 * (ny // 2 + 1), precision="f32")
+        p32.exec(x32.data_ptr(), out32.data_ptr())
+        ref = r64["out"]
+        x64d = x32.double()
+        p64 = hpxfft_amd.Plan(nx, ny, ld_in=2 * (ny // 2 + 1), precision="f64")
+        p64.exec(x64d.data_ptr(), ref.data_ptr())   # FP64 transform of the FP32-rounded input
+        torch.cuda.synchronize()
+        err = (torch.linalg.vector_norm(out32.to(torch.complex128) - ref) / torch.linalg.vector_norm(ref)).item()
+        p32.close()
+        p64.close()
+        result["f32"] = {"ms_per_step": round(r32["ms_per_step"], 4),
+                         "GBps": round(b32["total"] / (r32["ms_per_step"] * 1e-3) / 1e9, 2),
+                         "speedup_vs_f64": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
+                         "rel_l2_vs_f64": err,
+                         "gflops": round(nominal_flops(nx, ny) / (r32["ms_per_step"] * 1e-3) / 1e9, 1),
+                         "phases_s": {"row_r2c": r32["row_s"], "column_c2c": r32["col_s"]}}
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(nx, ny)
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
