@@ -209,6 +209,7 @@ const void *tw_ptr(const hpxfft_plan_s *p, size_t N)
     if (p->prec == P_F64) return p->tabs.tw64 + 2 * N;
     return p->tabs.tw32 + 2 * N;
 }
+const void *tw64_ptr(const hpxfft_plan_s *p, size_t N) { return p->tabs.tw64 + 2 * N; }
 
 std::string describe(const hpxfft_plan_s *p)
 {
@@ -235,9 +236,9 @@ std::string describe(const hpxfft_plan_s *p)
     }
     else
     {
-        const geom ga = col_geom(p->prec, p->lgQ), gb = col_geom(p->prec, p->lgP);
-        os << "  column c2c four-step Nx = P*Q = " << (1 << p->lgP) << " * " << (1 << p->lgQ) << ", chunks of "
-           << p->CW << " columns (scratch " << p->scratch_bytes << " B)\n";
+        const geom ga = tile_geom(p->prec, p->lgQ), gb = tile_geom(p->prec, p->lgP);
+        os << "  column c2c four-step Nx = P*Q = " << (1 << p->lgP) << " * " << (1 << p->lgQ) << " (scratch "
+           << p->scratch_bytes << " B, one block per strip)\n";
         os << "    phase A: " << (1 << p->lgQ) << "-point DFTs over rows m1+P*m2, twiddle w_Nx^(m1 k2), tiles of "
            << ga.nseq << " columns, " << ga.nt << " threads\n";
         os << "    phase B: " << (1 << p->lgP) << "-point DFTs, to rows Q*k1+k2, tiles of " << gb.nseq << " columns, "
@@ -263,7 +264,7 @@ int alloc_events(hpxfft_plan_s *p)
 int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj, int *col_exp)
 {
     const size_t sb = storage_bytes(p->prec);
-    const int W = p->W;
+    const int W = p->four_step ? tile_geom(p->prec, p->lgQ).nseq : p->W;
     ColArgs a{};
     a.pitch = (long long)p->ld_out;
     a.dist = (long long)p->dist_out;
@@ -276,8 +277,6 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
     {
         a.in = spec_in;
         a.out = spec_out;
-        a.c0 = 0;
-        a.CW = 0;
         a.P = 1;
         a.Q = (int)p->nx;
         a.tw_L = tw_ptr(p, p->nx);
@@ -289,7 +288,6 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
     const int P = 1 << p->lgP, Q = 1 << p->lgQ;
     a.P = P;
     a.Q = Q;
-    a.CW = (int)p->CW;
     a.tw_N = tw_ptr(p, p->nx);
     if (p->colmax)
     {
@@ -299,6 +297,7 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         HIPCHK(hipGetLastError());
         a.colmax = p->colmax;
     }
+    const unsigned gx = (unsigned)((p->ncols + W - 1) / W);
     for (size_t b = 0; b < p->batch; ++b)
     {
         const char *in_b = reinterpret_cast<const char *>(spec_in) + b * p->dist_out * 2 * sb;
@@ -308,22 +307,16 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         if (ab.row_exp) ab.row_exp += b * p->nx;
         if (ab.out_exp) ab.out_exp += b * p->ncols;
         if (ab.colmax) ab.colmax += b * p->ncols;
-        for (size_t c0 = 0; c0 < p->ncols; c0 += p->CW)
-        {
-            const size_t cols = std::min(p->CW, p->ncols - c0);
-            const unsigned gx = (unsigned)((cols + W - 1) / W);
-            ab.c0 = (int)c0;
-            // phase A: spectrum -> scratch
-            ab.in = in_b;
-            ab.out = p->scratch;
-            ab.tw_L = tw_ptr(p, (size_t)Q);
-            HIPCHK(col(p->prec, p->lgQ, 1, conj, ab, gx, (unsigned)P, 1, p->stream));
-            // phase B: scratch -> spectrum
-            ab.in = p->scratch;
-            ab.out = out_b;
-            ab.tw_L = tw_ptr(p, (size_t)P);
-            HIPCHK(col(p->prec, p->lgP, 2, conj, ab, gx, (unsigned)Q, 1, p->stream));
-        }
+        // phase A: spectrum -> scratch
+        ab.in = in_b;
+        ab.out = p->scratch;
+        ab.tw_L = tw_ptr(p, (size_t)Q);
+        HIPCHK(col(p->prec, p->lgQ, 1, conj, ab, gx, (unsigned)P, 1, p->stream));
+        // phase B: scratch -> spectrum
+        ab.in = p->scratch;
+        ab.out = out_b;
+        ab.tw_L = tw_ptr(p, (size_t)P);
+        HIPCHK(col(p->prec, p->lgP, 2, conj, ab, gx, (unsigned)Q, 1, p->stream));
     }
     return HPXFFT_OK;
 }
@@ -347,6 +340,7 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     r.dist_out = (long long)(2 * p->dist_out);
     r.nrows = (int)p->nx;
     r.tw = tw_ptr(p, p->ny);
+    r.tw64 = tw64_ptr(p, p->ny);
     r.in_exp = s16 ? in_exp : nullptr;
     r.out_exp = s16 ? p->mid_exp : nullptr;
     r.exp_dist = (long long)p->nx;
@@ -390,12 +384,10 @@ int plan_common(hpxfft_plan_s *p)
         if (p->lgP < COL_TILE_MIN_LG) { p->lgP = COL_TILE_MIN_LG; p->lgQ = p->lgx - p->lgP; }
         if (p->lgQ > COL_TILE_MAX_LG || p->lgP > COL_TILE_MAX_LG)
             return fail(HPXFFT_UNSUPPORTED_SIZE, "Nx too large");
-        // default chunk: scratch of about 128 MiB, a whole number of tiles
+        // scratch: one P x Q x TW block per strip of TW columns (one grid at a time)
         const size_t cb = compute_bytes(p->prec) * 2;
-        size_t cw = (size_t(128) << 20) / (p->nx * cb);
-        cw = std::max<size_t>(cw / p->W * p->W, (size_t)p->W);
-        const size_t full = (p->ncols + p->W - 1) / p->W * p->W;
-        p->CW = std::min(cw, full);
+        const size_t tw_cols = (size_t)tile_geom(p->prec, p->lgQ).nseq;
+        p->CW = (p->ncols + tw_cols - 1) / tw_cols * tw_cols;
         p->scratch_bytes = p->nx * p->CW * cb;
         HIPCHK(hipMalloc(&p->scratch, p->scratch_bytes));
     }
@@ -599,6 +591,7 @@ int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_rea
     r.dist_out = (long long)plan->dist_in;
     r.nrows = (int)plan->nx;
     r.tw = tw_ptr(plan, plan->ny);
+    r.tw64 = tw64_ptr(plan, plan->ny);
     const geom rg = row_geom(plan->prec, plan->lgL);
     const unsigned gx = (unsigned)((plan->nx + rg.nseq - 1) / rg.nseq);
     HIPCHK(row_c2r(plan->prec, plan->lgL, r, gx, (unsigned)plan->batch, plan->stream));

@@ -34,7 +34,7 @@ template <typename T, typename S> struct cfg
     {
         return (long)row_NSEQ(lg) * lp(1 << lg) * 2 * (long)sizeof(T) > lds_budget;
     }
-    // column tiles: W adjacent columns = one 128-byte row segment of storage
+    // direct column kernel: W adjacent columns = one 128-byte row segment of storage
     static constexpr int W = 128 / (2 * (int)sizeof(S)) > 32 ? 32 : 128 / (2 * (int)sizeof(S));
     static constexpr int col_direct_max_lg = sizeof(S) == 2 ? 9 : 10;
     static constexpr int col_E(int lg)
@@ -47,6 +47,16 @@ template <typename T, typename S> struct cfg
         // LDS layout [e][W]: padded only when a W-row is narrower than 128 B
         return (long)(W << lg) * 2 * (long)sizeof(T) > lds_budget;
     }
+    // four-step tiles: TW adjacent columns (512 B of FP64/FP32... storage per row
+    // segment for 8-byte elements, 256 B for 16-bit storage) x L rows
+    static constexpr int TW = sizeof(S) == 8 ? 32 : 64;
+    static constexpr int tile_E(int lg) { return 16; }
+    static constexpr int tile_NT(int lg) { return (TW << lg) / 16; }
+    static constexpr bool tile_SPLIT(int lg) { return (long)(TW << lg) * 2 * (long)sizeof(T) > lds_budget; }
+    template <int MODE> static constexpr int width() { return MODE == 0 ? W : TW; }
+    template <int MODE> static constexpr int nt(int lg) { return MODE == 0 ? col_NT(lg) : tile_NT(lg); }
+    template <int MODE> static constexpr int ept(int lg) { return MODE == 0 ? col_E(lg) : tile_E(lg); }
+    template <int MODE> static constexpr bool split(int lg) { return MODE == 0 ? col_SPLIT(lg) : tile_SPLIT(lg); }
 };
 
 struct RowArgs
@@ -59,6 +69,7 @@ struct RowArgs
     long long dist_out;
     int nrows;
     const void *tw;        // w_{2L}^m, m < 2L  (compute precision)
+    const void *tw64;      // the same table in FP64 (twiddle products of FP32 rows are formed in FP64)
     const int *in_exp;     // optional per-row input exponent (value = stored * 2^-e)
     int *out_exp;          // 16-bit outputs: per-row exponent written here
     long long exp_dist;    // between grids, for in_exp/out_exp
@@ -71,9 +82,7 @@ struct ColArgs
     long long pitch;   // complex elements between spectrum rows
     long long dist;    // complex elements between spectrum grids (batch)
     int ncols;         // valid columns of the spectrum (Ny/2+1)
-    int c0;            // first column of this launch (chunk start)
-    int CW;            // scratch chunk width (columns)
-    int P, Q;          // four-step split Nx = P*Q
+    int P, Q;          // four-step split Nx = P*Q; scratch index ((strip*Q + k2)*P + m1)*TW + w
     const void *tw_L;  // w_L^m, m < L (sub-FFT length)
     const void *tw_N;  // w_Nx^m, m < Nx (MODE 1 post-twiddle)
     const int *row_exp;    // 16-bit inputs (MODE 0,1): per-row exponent of the rows
@@ -245,13 +254,13 @@ row_r2c_kernel(RowArgs a)
 // column tile kernel.  blockIdx.x = strip of W columns inside the launch's
 // chunk, blockIdx.y = m1 (MODE 1) / k2 (MODE 2) / 0, blockIdx.z = grid.
 template <typename T, typename SI, typename SO, typename SC, int LG, int MODE, bool CONJ>
-__global__ void __launch_bounds__((cfg<T, SC>::col_NT(LG)))
+__global__ void __launch_bounds__((cfg<T, SC>::template nt<MODE>(LG)))
 col_kernel(ColArgs a)
 {
     constexpr int L = 1 << LG;
     using C = cfg<T, SC>;   // SC = spectrum storage type: fixes the tile width W
-    constexpr int W = C::W, E = C::col_E(LG);
-    constexpr bool SPLIT = C::col_SPLIT(LG);
+    constexpr int W = C::template width<MODE>(), E = C::template ept<MODE>(LG);
+    constexpr bool SPLIT = C::template split<MODE>(LG);
     using EN = Engine<T, L, E, W, true, SPLIT>;
 
     __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
@@ -261,9 +270,10 @@ col_kernel(ColArgs a)
     const int t = threadIdx.x;
     const long long b = blockIdx.z;
     const int sub = blockIdx.y;
-    const int col0 = a.c0 + blockIdx.x * W;
+    const int strip = blockIdx.x;
+    const int col0 = strip * W;
     const int nvalid = min(W, a.ncols - col0);
-    const int lc0 = col0 - a.c0;   // column inside the scratch chunk
+    const long long sbase = (long long)strip * a.P * a.Q * W;   // this strip's scratch block
 
     const SI *in = reinterpret_cast<const SI *>(a.in);
     SO *out = reinterpret_cast<SO *>(a.out);
@@ -283,7 +293,7 @@ col_kernel(ColArgs a)
             int row = 0;
             if constexpr (MODE == 0) { row = e; idx = (long long)row * a.pitch + col0 + w; }
             else if constexpr (MODE == 1) { row = sub + a.P * e; idx = (long long)row * a.pitch + col0 + w; }
-            else { idx = ((long long)sub * a.P + e) * a.CW + lc0 + w; }
+            else { idx = sbase + ((long long)sub * a.P + e) * W + w; }
             z = stor<SI>::ld(in + 2 * idx);
             if constexpr (is16<SI>() && MODE != 2)
             {
@@ -339,7 +349,7 @@ col_kernel(ColArgs a)
             cpx<T> z = v[x];
             long long idx;
             if constexpr (MODE == 0) idx = (long long)e * a.pitch + col0 + w;
-            else if constexpr (MODE == 1) idx = ((long long)e * a.P + sub) * a.CW + lc0 + w;
+            else if constexpr (MODE == 1) idx = sbase + ((long long)e * a.P + sub) * W + w;
             else idx = ((long long)a.Q * e + sub) * a.pitch + col0 + w;
             if constexpr (CONJ && MODE != 1) z = conjc(z);
             if constexpr (OUT16)

@@ -1,0 +1,259 @@
+// fft_row_big.hpp -- row r2c kernel for long rows, one row per workgroup,
+// L/16 threads with 16 points each (L = Ny/2 complex points, L = 2^11, 2^13
+// or 2^14: 16^k times a final radix RL of 8, 2 or 4).
+//
+// Stages: radix-16 Stockham stages in VGPRs with one LDS exchange each, then
+// the final radix-RL stage fused with the r2c split.  The thread that owns
+// butterflies j and NSL-j of the final stage holds both Z[k] and Z[L-k] for
+// every output it produces, so the split needs no further LDS round trip.
+// LDS addresses are pad(e) = e + e/16; each exchange address is a per-thread
+// base plus a compile-time stride (immediate offsets).  FP64 rows exchange
+// real and imaginary halves separately so the LDS footprint is 8 bytes per
+// point (two workgroups per CU at L = 8192).
+#pragma once
+
+#include "fft_device.hpp"
+
+namespace hpxfft_dev {
+
+template <typename T, int L> struct bigrow
+{
+    static constexpr int LG = ilog2c(L);
+    static constexpr int NT = L / 16;
+    static constexpr int NR16 = LG / 4;                // radix-16 stages before the fused one
+    static constexpr int RL = 1 << (LG % 4);           // radix of the fused final stage
+    static constexpr int NSL = L / RL;                 // its Ns
+    static constexpr int HALF = NSL / 2;               // fused-stage tasks j = 0..HALF
+    static constexpr int NMAIN = HALF / NT;            // regular tasks per thread (j = t + i NT < HALF)
+    static constexpr int LP = L + L / 16;
+    // exchange real/imag halves when a complex row would take more than 40 KiB of LDS
+    static constexpr bool SPLIT = (long)LP * 2 * (long)sizeof(T) > 40 * 1024;
+    static constexpr int ns(int s) { return s == 0 ? 1 : 16 * ns(s - 1); }
+    __device__ static __forceinline__ int pad(int e) { return e + (e >> 4); }
+    static_assert(LG % 4 != 0 && NR16 >= 2 && NMAIN >= 1 && HALF % NT == 0, "supported: L = 2^11, 2^13, 2^14");
+};
+
+// stage twiddles w_{16 NS}^{r k}, r = 1..15, from the FP64 w_{2L} table:
+// table values w, w^2, w^4, w^8 and products of at most 3 factors, formed in
+// FP64 and (for FP32 rows) rounded once -- the only FP64 step of the FP32 path.
+template <typename T, int L, int NS>
+__device__ __forceinline__ void twiddle16(cpx<T> *v, const cpx<double> *__restrict__ tw64, int k)
+{
+    const int base = k * (L / (8 * NS));   // index of w_{16 NS}^k in the w_{2L} table
+    const cpx<double> w1 = tw64[base], w2 = tw64[2 * base], w4 = tw64[4 * base], w8 = tw64[8 * base];
+    const cpx<double> w3 = cmul(w2, w1), w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
+    const cpx<double> ws[16] = {{1.0, 0.0},      w1,           w2,           w3,           w4,           w5,
+                                w6,              w7,           w8,           cmul(w8, w1), cmul(w8, w2), cmul(w8, w3),
+                                cmul(w8, w4),    cmul(w8, w5), cmul(w8, w6), cmul(w8, w7)};
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], cpx<T>{(T)ws[r].x, (T)ws[r].y});
+}
+
+template <typename T, int L, int S>
+__device__ __forceinline__ void radix16_stages(cpx<T> *v, T *lds, const cpx<double> *__restrict__ tw64, int t)
+{
+    using B = bigrow<T, L>;
+    // stages 0 .. NR16-2 (each followed by an exchange); stage NR16-1 is
+    // computed by the caller and handed to the fused stage
+    if constexpr (S + 1 < B::NR16)
+    {
+        constexpr int NS = B::ns(S);
+        const int k = t & (NS - 1);
+        if constexpr (NS > 1) twiddle16<T, L, NS>(v, tw64, k);
+        dft16(v);
+        {
+            // outputs e_q = (t/NS) 16 NS + k + q NS -> padded base + q*WS ; inputs t + r NT
+            constexpr int WS = NS == 1 ? 1 : NS + NS / 16;
+            constexpr int RS = B::NT + B::NT / 16;
+            const int wb = B::pad((t / NS) * 16 * NS + k), rb = B::pad(t);
+            if constexpr (!B::SPLIT)
+            {
+                cpx<T> *l = reinterpret_cast<cpx<T> *>(lds);
+#pragma unroll
+                for (int q = 0; q < 16; ++q) l[wb + q * WS] = v[q];
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[r] = l[rb + r * RS];
+                __syncthreads();
+            }
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) lds[wb + q * WS] = v[q].x;
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[r].x = lds[rb + r * RS];
+                __syncthreads();
+#pragma unroll
+                for (int q = 0; q < 16; ++q) lds[wb + q * WS] = v[q].y;
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[r].y = lds[rb + r * RS];
+                __syncthreads();
+            }
+            radix16_stages<T, L, S + 1>(v, lds, tw64, t);
+        }
+    }
+}
+
+// gather the fused-stage inputs: for task j (butterflies j and NSL-j),
+// a[r] = d[j + r NSL], c[r] = d[NSL-j + r NSL]
+template <typename T, int L, bool HALFSEL>
+__device__ __forceinline__ void gather_fused(T *lds, const cpx<T> *v, cpx<T> (&a)[bigrow<T, L>::NMAIN][bigrow<T, L>::RL],
+                                             cpx<T> (&c)[bigrow<T, L>::NMAIN][bigrow<T, L>::RL],
+                                             cpx<T> (&s)[bigrow<T, L>::RL], int t)
+{
+    using B = bigrow<T, L>;
+    constexpr int RL = B::RL, NSL = B::NSL, NT = B::NT;
+    constexpr int NS = B::ns(B::NR16 - 1);
+    constexpr int WS = NS == 1 ? 1 : NS + NS / 16;
+    constexpr int RS = NSL + NSL / 16;
+    const int k = t & (NS - 1);
+    const int wb = B::pad((t / NS) * 16 * NS + k);
+    // write this thread's last radix-16 outputs (one component when split)
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+    {
+        if constexpr (!B::SPLIT) reinterpret_cast<cpx<T> *>(lds)[wb + q * WS] = v[q];
+        else lds[wb + q * WS] = HALFSEL ? v[q].y : v[q].x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < B::NMAIN; ++i)
+    {
+        const int j = t + i * NT;   // < HALF
+        const int pa = B::pad(j), pc = B::pad(NSL - j);
+#pragma unroll
+        for (int r = 0; r < RL; ++r)
+        {
+            const int ia = pa + r * RS, ic = (j == 0 ? pa : pc) + r * RS;
+            if constexpr (!B::SPLIT)
+            {
+                a[i][r] = reinterpret_cast<const cpx<T> *>(lds)[ia];
+                c[i][r] = reinterpret_cast<const cpx<T> *>(lds)[ic];
+            }
+            else if constexpr (HALFSEL)
+            {
+                a[i][r].y = lds[ia];
+                c[i][r].y = lds[ic];
+            }
+            else
+            {
+                a[i][r].x = lds[ia];
+                c[i][r].x = lds[ic];
+            }
+        }
+    }
+    if (t == B::HALF % NT)
+    {
+        const int ps = B::pad(B::HALF);
+#pragma unroll
+        for (int r = 0; r < RL; ++r)
+        {
+            if constexpr (!B::SPLIT) s[r] = reinterpret_cast<const cpx<T> *>(lds)[ps + r * RS];
+            else if constexpr (HALFSEL) s[r].y = lds[ps + r * RS];
+            else s[r].x = lds[ps + r * RS];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, typename SO>
+__device__ __forceinline__ void r2c_pair(SO *orow, int L, int k, cpx<T> A, cpx<T> Bc, const cpx<T> *__restrict__ tw)
+{
+    // X[k] = Fe + w_{2L}^k Fo, X[L-k] = conj(Fe - w_{2L}^k Fo)
+    const cpx<T> Bk = conjc(Bc);
+    const cpx<T> Fe = scal(A + Bk, T(0.5));
+    const cpx<T> D = scal(A - Bk, T(0.5));
+    const cpx<T> Fo = {D.y, -D.x};
+    const cpx<T> tt = cmul(tw[k], Fo);
+    stor<SO>::st(orow + 2 * k, Fe + tt);
+    stor<SO>::st(orow + 2 * (L - k), conjc(Fe - tt));
+}
+
+// Transform one row held in v[] (stage-0 inputs: v[r] = z[t + r NT]) and
+// store its Ny/2+1 outputs to orow.
+template <typename T, int LG>
+__device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__restrict__ tw,
+                                             const cpx<double> *__restrict__ tw64, T *orow, int t)
+{
+    constexpr int L = 1 << LG;
+    using B = bigrow<T, L>;
+    constexpr int NT = B::NT, RL = B::RL, NSL = B::NSL, HALF = B::HALF;
+
+    radix16_stages<T, L, 0>(v, lds, tw64, t);
+    {
+        // last radix-16 stage; its outputs are handed to the fused stage through LDS
+        constexpr int NS = B::ns(B::NR16 - 1);
+        const int k = t & (NS - 1);
+        if constexpr (NS > 1) twiddle16<T, L, NS>(v, tw64, k);
+        dft16(v);
+    }
+    cpx<T> a[B::NMAIN][RL], c[B::NMAIN][RL], s[RL];
+    gather_fused<T, L, false>(lds, v, a, c, s, t);
+    if constexpr (B::SPLIT) gather_fused<T, L, true>(lds, v, a, c, s, t);
+
+#pragma unroll
+    for (int i = 0; i < B::NMAIN; ++i)
+    {
+        const int j = t + i * NT;
+        const int jp = NSL - j;   // == NSL for j == 0 (then c == a)
+        // final-stage twiddles w_L^{r b} = w_{2L}^{2 r b}
+#pragma unroll
+        for (int r = 1; r < RL; ++r)
+        {
+            a[i][r] = cmul(a[i][r], tw[2 * r * j]);
+            c[i][r] = cmul(c[i][r], tw[(2 * r * jp) & (2 * L - 1)]);
+        }
+        dftR<T, RL>(a[i]);
+        dftR<T, RL>(c[i]);
+        // a[q] = Z[j + q NSL], c[q] = Z[jp + q NSL]
+        if (j == 0)
+        {
+            const cpx<T> z0 = a[i][0];
+            stor<T>::st(orow, cpx<T>{z0.x + z0.y, T(0)});
+            stor<T>::st(orow + 2 * L, cpx<T>{z0.x - z0.y, T(0)});
+#pragma unroll
+            for (int q = 1; q < RL / 2; ++q) r2c_pair<T, T>(orow, L, q * NSL, a[i][q], a[i][RL - q], tw);
+            stor<T>::st(orow + 2 * (L / 2), conjc(a[i][RL / 2]));
+        }
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < RL; ++q) r2c_pair<T, T>(orow, L, j + q * NSL, a[i][q], c[i][RL - 1 - q], tw);
+        }
+    }
+    if (t == HALF % NT)
+    {
+        // self-paired butterfly j = HALF: pairs (q, RL-1-q)
+#pragma unroll
+        for (int r = 1; r < RL; ++r) s[r] = cmul(s[r], tw[2 * r * HALF]);
+        dftR<T, RL>(s);
+#pragma unroll
+        for (int q = 0; q < RL / 2; ++q) r2c_pair<T, T>(orow, L, HALF + q * NSL, s[q], s[RL - 1 - q], tw);
+    }
+}
+
+// Forward r2c of long rows (FP64 / FP32 storage = compute type): workgroup
+// g transforms row g of the nrows x batch rows.
+template <typename T, int LG>
+__global__ void __launch_bounds__((1 << LG) / 16)
+row_r2c_big_kernel(const T *__restrict__ in, T *__restrict__ out, long long ld_in, long long ld_out,
+                   long long dist_in, long long dist_out, int nrows, const cpx<T> *__restrict__ tw,
+                   const cpx<double> *__restrict__ tw64)
+{
+    constexpr int L = 1 << LG;
+    using B = bigrow<T, L>;
+    constexpr int NT = B::NT;
+    __shared__ T lds[B::SPLIT ? B::LP : 2 * B::LP];
+    const int t = threadIdx.x;
+    const int g = blockIdx.x;
+    const int b = g / nrows, row = g - b * nrows;
+    const T *irow = in + (long long)b * dist_in + (long long)row * ld_in;
+    cpx<T> v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = stor<T>::ld(irow + 2 * (t + r * NT));
+    big_row_body<T, LG>(v, lds, tw, tw64, out + (long long)b * dist_out + (long long)row * ld_out, t);
+}
+
+}   // namespace hpxfft_dev

@@ -1,0 +1,210 @@
+// fft_tile.hpp -- four-step column tiles for FP64 / FP32 storage.
+//
+// A tile is W adjacent columns (one 512-byte row segment for FP64) of
+// L = 16*R1 rows (L = 32..256), transformed by W*L/16 threads with 16 points
+// each: a radix-16 stage in VGPRs, one LDS exchange, then R1-point stages
+// (16/R1 butterflies per thread).  LDS layout [e][w]; consecutive threads
+// take consecutive columns, so every LDS and global access of a wave covers
+// whole row segments.
+//   MODE 1 (phase A): rows m1 + P*e of the spectrum -> e-th DFT output k2,
+//          times w_Nx^{m1 k2} -> scratch ((strip*Q + k2)*P + m1)*W + w
+//   MODE 2 (phase B): scratch ((strip*Q + k2)*P + e)*W + w -> DFT over m1 ->
+//          spectrum row Q*k1 + k2
+// CONJ computes the inverse transform as conj(DFT(conj(.))).
+#pragma once
+
+#include "fft_kernels.hpp"
+
+namespace hpxfft_dev {
+
+// twiddles w^{r j}, r = 1..R-1 (R <= 16) applied to a[1..R-1]; w^m = tab[m*step]
+template <typename T, int R>
+__device__ __forceinline__ void twiddle_r(cpx<T> *a, const cpx<T> *__restrict__ tab, int jstep)
+{
+    if constexpr (sizeof(T) == 8 && R >= 8)
+    {
+        // FP64: w, w^2, w^4 (and w^8) from the table, the rest as products
+        const cpx<T> w1 = tab[jstep], w2 = tab[2 * jstep], w4 = tab[4 * jstep];
+        const cpx<T> w3 = cmul(w2, w1);
+        a[1] = cmul(a[1], w1);
+        a[2] = cmul(a[2], w2);
+        a[3] = cmul(a[3], w3);
+        a[4] = cmul(a[4], w4);
+        a[5] = cmul(a[5], cmul(w4, w1));
+        a[6] = cmul(a[6], cmul(w4, w2));
+        a[7] = cmul(a[7], cmul(w4, w3));
+        if constexpr (R == 16)
+        {
+            const cpx<T> w8 = tab[8 * jstep];
+            a[8] = cmul(a[8], w8);
+#pragma unroll
+            for (int r = 9; r < 16; ++r)
+            {
+                const int lo = r - 8;
+                const cpx<T> wl = lo == 1 ? w1 : lo == 2 ? w2 : lo == 3 ? w3 : lo == 4 ? w4
+                                  : lo == 5 ? cmul(w4, w1) : lo == 6 ? cmul(w4, w2) : cmul(w4, w3);
+                a[r] = cmul(a[r], cmul(w8, wl));
+            }
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int r = 1; r < R; ++r) a[r] = cmul(a[r], tab[r * jstep]);
+    }
+}
+
+template <typename T, int L, int W> struct tileinfo
+{
+    static constexpr int R1 = L / 16;        // second-stage radix (2, 4, 8, 16)
+    static constexpr int NT = W * L / 16;
+    static constexpr int JPC = L / 16;       // threads per column
+    static constexpr int NB1 = 16 / R1;      // second-stage butterflies per thread
+    // FP64 tiles exchange real and imaginary halves (half the LDS, twice the workgroups per CU)
+    static constexpr bool SPLIT = sizeof(T) == 8;
+};
+
+template <typename T, typename SC, int LG, int MODE, bool CONJ>
+__global__ void __launch_bounds__((cfg<T, SC>::TW << LG) / 16)
+tile_kernel(ColArgs a)
+{
+    constexpr int L = 1 << LG;
+    constexpr int W = cfg<T, SC>::TW;
+    using TI = tileinfo<T, L, W>;
+    constexpr int R1 = TI::R1, JPC = TI::JPC, NB1 = TI::NB1;
+    static_assert(R1 >= 2 && R1 <= 16, "L = 32..256");
+    constexpr bool SPLIT = TI::SPLIT;
+    __shared__ T lds[SPLIT ? L * W : 2 * L * W];
+
+    const int t = threadIdx.x;
+    const int w = t % W, jj = t / W;   // column in the tile, butterfly index in the column
+    const int sub = blockIdx.y;        // m1 (MODE 1) or k2 (MODE 2)
+    const int strip = blockIdx.x;
+    const int col0 = strip * W;
+    const bool valid = w < a.ncols - col0;
+    const long long sbase = (long long)strip * a.P * a.Q * W;
+    const cpx<T> *twL = reinterpret_cast<const cpx<T> *>(a.tw_L);
+
+    // ---- load: element e = jj + r*JPC of column w (stage-0 butterfly jj)
+    cpx<T> v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+    {
+        const int e = jj + r * JPC;
+        long long idx;
+        if constexpr (MODE == 1) idx = (long long)(sub + a.P * e) * a.pitch + col0 + w;
+        else idx = sbase + ((long long)sub * a.P + e) * W + w;
+        cpx<T> z = {T(0), T(0)};
+        if (MODE == 2 || valid) z = stor<T>::ld(reinterpret_cast<const T *>(a.in) + 2 * idx);
+        if constexpr (CONJ && MODE == 1) z = conjc(z);
+        v[r] = z;
+    }
+    // ---- stage 0: radix 16, Ns = 1; outputs at e = 16 jj + q
+    dft16(v);
+    // ---- exchange; stage 1 (radix R1, Ns = 16): butterfly j1 = jj + i*JPC reads e = j1 + 16 r
+    if constexpr (!SPLIT)
+    {
+        cpx<T> *l = reinterpret_cast<cpx<T> *>(lds);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) l[(16 * jj + q) * W + w] = v[q];
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NB1; ++i)
+        {
+            const int j1 = jj + i * JPC;
+#pragma unroll
+            for (int r = 0; r < R1; ++r) v[i * R1 + r] = l[(j1 + 16 * r) * W + w];
+        }
+    }
+    else
+    {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) lds[(16 * jj + q) * W + w] = v[q].x;
+        __syncthreads();
+        T re[16];
+#pragma unroll
+        for (int i = 0; i < NB1; ++i)
+        {
+            const int j1 = jj + i * JPC;
+#pragma unroll
+            for (int r = 0; r < R1; ++r) re[i * R1 + r] = lds[(j1 + 16 * r) * W + w];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 16; ++q) lds[(16 * jj + q) * W + w] = v[q].y;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NB1; ++i)
+        {
+            const int j1 = jj + i * JPC;
+#pragma unroll
+            for (int r = 0; r < R1; ++r) v[i * R1 + r] = {re[i * R1 + r], lds[(j1 + 16 * r) * W + w]};
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NB1; ++i)
+    {
+        const int j1 = jj + i * JPC;
+        twiddle_r<T, R1>(v + i * R1, twL, j1);   // w_L^{r j1}
+        dftR<T, R1>(v + i * R1);
+    }
+    // outputs: butterfly j1, q -> e = j1 + 16 q
+    if constexpr (MODE == 1)
+    {
+        // post-twiddle w_Nx^{m1 e}, e = j1 + 16 q
+        if (sub != 0)
+        {
+            const cpx<T> *twN = reinterpret_cast<const cpx<T> *>(a.tw_N);
+            const int nmask = a.P * a.Q - 1;
+            if constexpr (sizeof(T) == 8)
+            {
+                // FP64: w^{m1 j1} (per lane) * w^{16 m1 q} (uniform across the workgroup), both table values
+#pragma unroll
+                for (int i = 0; i < NB1; ++i)
+                {
+                    const int j1 = jj + i * JPC;
+                    const cpx<T> wb = twN[sub * j1];
+#pragma unroll
+                    for (int q = 0; q < R1; ++q)
+                    {
+                        const cpx<T> wq = twN[(16 * sub * q) & nmask];
+                        v[i * R1 + q] = cmul(v[i * R1 + q], q == 0 ? wb : cmul(wb, wq));
+                    }
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int i = 0; i < NB1; ++i)
+                {
+                    const int j1 = jj + i * JPC;
+#pragma unroll
+                    for (int q = 0; q < R1; ++q) v[i * R1 + q] = cmul(v[i * R1 + q], twN[sub * (j1 + 16 * q)]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < NB1; ++i)
+    {
+        const int j1 = jj + i * JPC;
+#pragma unroll
+        for (int q = 0; q < R1; ++q)
+        {
+            const int e = j1 + 16 * q;
+            cpx<T> z = v[i * R1 + q];
+            if constexpr (MODE == 1)
+            {
+                stor<T>::st(reinterpret_cast<T *>(a.out) + 2 * (sbase + ((long long)e * a.P + sub) * W + w), z);
+            }
+            else
+            {
+                if constexpr (CONJ) z = conjc(z);
+                if (valid)
+                    stor<T>::st(reinterpret_cast<T *>(a.out) + 2 * ((long long)(a.Q * e + sub) * a.pitch + col0 + w), z);
+            }
+        }
+    }
+}
+
+}   // namespace hpxfft_dev

@@ -2,6 +2,9 @@
 #pragma once
 
 #include "fft_kernels.hpp"
+#include "fft_row_big.hpp"
+#include "fft_tile.hpp"
+#include <algorithm>
 
 namespace hpxfft_host {
 
@@ -30,6 +33,7 @@ struct geom
                          hipStream_t s);                                                                           \
     geom row_geom_##sfx(int lg);                                                                                   \
     geom col_geom_##sfx(int lg);                                                                                   \
+    geom tile_geom_##sfx(int lg);                                                                                  \
     int col_direct_max_lg_##sfx();
 
 HPXFFT_DECL_LAUNCH(f64)
@@ -87,6 +91,16 @@ inline geom col_geom(int p, int lg)
     default: return col_geom_bf16(lg);
     }
 }
+inline geom tile_geom(int p, int lg)
+{
+    switch (p)
+    {
+    case P_F64: return tile_geom_f64(lg);
+    case P_F32: return tile_geom_f32(lg);
+    case P_F16: return tile_geom_f16(lg);
+    default: return tile_geom_bf16(lg);
+    }
+}
 inline int col_direct_max_lg(int p)
 {
     switch (p)
@@ -107,12 +121,29 @@ namespace hpxfft_host {
 namespace detail {
 using namespace hpxfft_dev;
 
+template <typename T, typename S, int LG> constexpr bool use_big_row()
+{
+    return std::is_same<T, S>::value && (LG == 11 || LG == 13 || LG == 14);
+}
 template <typename T, typename S, int LG>
 hipError_t launch_row_r2c(const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
 {
-    hipLaunchKernelGGL((row_r2c_kernel<T, S, S, LG>), dim3(gx, gy), dim3(cfg<T, S>::row_NT(LG)), 0, s, a);
+    if constexpr (use_big_row<T, S, LG>())
+    {
+        (void)gx;
+        const unsigned total = (unsigned)a.nrows * gy;
+        hipLaunchKernelGGL((row_r2c_big_kernel<T, LG>), dim3(total), dim3((1 << LG) / 16), 0, s,
+                           reinterpret_cast<const T *>(a.in), reinterpret_cast<T *>(a.out), a.ld_in, a.ld_out,
+                           a.dist_in, a.dist_out, a.nrows, reinterpret_cast<const cpx<T> *>(a.tw),
+                           reinterpret_cast<const cpx<double> *>(a.tw64));
+    }
+    else
+    {
+        hipLaunchKernelGGL((row_r2c_kernel<T, S, S, LG>), dim3(gx, gy), dim3(cfg<T, S>::row_NT(LG)), 0, s, a);
+    }
     return hipGetLastError();
 }
+
 template <typename T, typename S, int LG>
 hipError_t launch_row_c2r(const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
 {
@@ -122,11 +153,21 @@ hipError_t launch_row_c2r(const RowArgs &a, unsigned gx, unsigned gy, hipStream_
 template <typename T, typename S, int LG, int MODE, bool CONJ>
 hipError_t launch_col(const ColArgs &a, unsigned gx, unsigned gy, unsigned gz, hipStream_t s)
 {
-    // MODE 1 writes the compute-precision scratch, MODE 2 reads it
-    using SI = typename std::conditional<MODE == 2, T, S>::type;
-    using SO = typename std::conditional<MODE == 1, T, S>::type;
-    hipLaunchKernelGGL((col_kernel<T, SI, SO, S, LG, MODE, CONJ>), dim3(gx, gy, gz), dim3(cfg<T, S>::col_NT(LG)), 0,
-                       s, a);
+    if constexpr (MODE != 0 && std::is_same<T, S>::value)
+    {
+        // four-step tiles, FP64 / FP32 storage: specialised two-stage kernel
+        (void)gz;
+        hipLaunchKernelGGL((tile_kernel<T, S, LG, MODE, CONJ>), dim3(gx, gy), dim3((cfg<T, S>::TW << LG) / 16), 0, s,
+                           a);
+    }
+    else
+    {
+        // MODE 1 writes the compute-precision scratch, MODE 2 reads it
+        using SI = typename std::conditional<MODE == 2, T, S>::type;
+        using SO = typename std::conditional<MODE == 1, T, S>::type;
+        hipLaunchKernelGGL((col_kernel<T, SI, SO, S, LG, MODE, CONJ>), dim3(gx, gy, gz),
+                           dim3(cfg<T, S>::template nt<MODE>(LG)), 0, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -185,12 +226,19 @@ hipError_t row_c2r_maybe(int lg, const RowArgs &a, unsigned gx, unsigned gy, hip
 template <typename T, typename S> geom row_geom_t(int lg)
 {
     using C = cfg<T, S>;
+    if (std::is_same<T, S>::value && (lg == 11 || lg == 13 || lg == 14))
+        return geom{(1 << lg) / 16, 1, 16, (long)((1 << lg) + (1 << lg) / 16) * 2 * (long)sizeof(T) > 40 * 1024};
     return geom{C::row_NT(lg), C::row_NSEQ(lg), C::row_E(lg), C::row_SPLIT(lg)};
 }
 template <typename T, typename S> geom col_geom_t(int lg)
 {
     using C = cfg<T, S>;
     return geom{C::col_NT(lg), C::W, C::col_E(lg), C::col_SPLIT(lg)};
+}
+template <typename T, typename S> geom tile_geom_t(int lg)
+{
+    using C = cfg<T, S>;
+    return geom{C::tile_NT(lg), C::TW, C::tile_E(lg), C::tile_SPLIT(lg)};
 }
 }   // namespace detail
 }   // namespace hpxfft_host
@@ -212,6 +260,7 @@ template <typename T, typename S> geom col_geom_t(int lg)
     }                                                                                                              \
     geom row_geom_##sfx(int lg) { return detail::row_geom_t<T, S>(lg); }                                           \
     geom col_geom_##sfx(int lg) { return detail::col_geom_t<T, S>(lg); }                                           \
+    geom tile_geom_##sfx(int lg) { return detail::tile_geom_t<T, S>(lg); }                                         \
     int col_direct_max_lg_##sfx() { return hpxfft_dev::cfg<T, S>::col_direct_max_lg; }                             \
     }
 #endif
