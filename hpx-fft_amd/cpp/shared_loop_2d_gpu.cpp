@@ -1,0 +1,131 @@
+// shared_loop_2d_gpu -- the harness of HPX-FFT's examples/hpxfft/shared_loop_2d.cpp
+// driving hpxfft::shared::loop, here resolved to the MI355X class.
+//
+// Same command line (shared_loop_2d.cpp:140-146): --nx (default 8) --ny
+// (default 14) --plan (estimate) --run (par|seq) --header (0) --result (0),
+// given as --key=value or --key value.  Same ramp input x[i][j] = j (:33-40),
+// same console report (:69-93), same ';'-separated CSV appended to
+// runtimes/runtimes_hpx_shared_loop.txt (:95-111), same plan file
+// plans/plan_hpx_shared_loop.txt (:114-128).  The n_threads column holds the
+// string "gpu".  Sizes must be powers of two (otherwise the class throws
+// std::invalid_argument and the program exits with status 2).
+#include "hpxfft/shared/loop.hpp"
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <filesystem>
+#include <fstream>
+#include <iostream>
+#include <map>
+#include <string>
+
+static void create_parent_dir(const std::string &path)
+{
+    const std::filesystem::path p(path);
+    if (p.has_parent_path()) std::filesystem::create_directories(p.parent_path());
+}
+
+static void print_vector_2d(const hpxfft::shared::vector_2d &v)
+{
+    std::size_t counter = 0;
+    for (std::size_t i = 0; i < v.n_row(); ++i)
+    {
+        for (std::size_t j = 0; j < v.n_col(); ++j)
+        {
+            if (counter % 2 == 0) std::cout << "(" << v(i, j) << " ";
+            else std::cout << v(i, j) << ") ";
+            ++counter;
+        }
+        std::cout << "\n";
+    }
+    std::cout << std::endl;
+}
+
+int main(int argc, char *argv[])
+{
+    std::map<std::string, std::string> opt = {{"result", "0"}, {"nx", "8"},      {"ny", "14"},
+                                              {"plan", "estimate"}, {"run", "par"}, {"header", "0"}};
+    for (int i = 1; i < argc; ++i)
+    {
+        std::string a = argv[i];
+        if (a.rfind("--", 0) != 0) continue;
+        a = a.substr(2);
+        const auto eq = a.find('=');
+        if (eq != std::string::npos) opt[a.substr(0, eq)] = a.substr(eq + 1);
+        else if (i + 1 < argc) opt[a] = argv[++i];
+    }
+    const std::string run_flag = opt["run"], plan_flag = opt["plan"];
+    const bool print_result = std::stoi(opt["result"]) != 0, print_header = std::stoi(opt["header"]) != 0;
+    const std::size_t dim_c_x = std::stoul(opt["nx"]);
+    const std::size_t dim_r_y = std::stoul(opt["ny"]);
+    const std::size_t dim_c_y = dim_r_y / 2 + 1;
+
+    using clock = std::chrono::steady_clock;
+    hpxfft::shared::vector_2d values_vec(dim_c_x, 2 * dim_c_y);
+    for (std::size_t i = 0; i < dim_c_x; ++i)
+        for (std::size_t j = 0; j < dim_r_y; ++j) values_vec(i, j) = (double)j;
+
+    hpxfft::shared::loop fft_computer;
+    double total = 0.0, init = 0.0;
+    try
+    {
+        const auto start_total = clock::now();
+        fft_computer.initialize(std::move(values_vec), plan_flag);
+        const auto stop_init = clock::now();
+        if (run_flag == "seq") values_vec = fft_computer.fft_2d_r2c_seq();
+        else values_vec = fft_computer.fft_2d_r2c_par();
+        const auto stop_total = clock::now();
+        total = std::chrono::duration<double>(stop_total - start_total).count();
+        init = std::chrono::duration<double>(stop_init - start_total).count();
+    }
+    catch (const std::invalid_argument &e)
+    {
+        std::cerr << "invalid argument: " << e.what() << std::endl;
+        return 2;
+    }
+    catch (const std::exception &e)
+    {
+        std::cerr << "error: " << e.what() << std::endl;
+        return 1;
+    }
+    if (print_result) print_vector_2d(values_vec);
+
+    auto m = [&](const char *k) { return fft_computer.get_measurement(k); };
+    std::cout << "\nLocality 0 - shared - " << run_flag << "\n"
+              << "Total runtime : " << total << "\n"
+              << "Initialization: " << init << "\n"
+              << "FFT 2D runtime: " << m("total") << "\n"
+              << "FFTW r2c      : " << m("first_fftw") << "\n"
+              << "First trans   : " << m("first_trans") << "\n"
+              << "FFTW c2c      : " << m("second_fftw") << "\n"
+              << "Second trans  : " << m("second_trans") << "\n"
+              << "Plan time     : " << m("plan") << "\n"
+              << "Plan flops    : " << m("plan_flops") << "\n"
+              << std::flush;
+
+    const std::string header = "n_threads;n_x;n_y;plan;run_flag;total;initialization;fft_2d_total;first_fftw;"
+                               "first_trans;second_fftw;second_trans;plan_time;plan_flops;\n";
+    auto line = [&]() {
+        return std::string("gpu;") + std::to_string(dim_c_x) + ";" + std::to_string(dim_r_y) + ";" + plan_flag + ";" +
+               run_flag + ";" + std::to_string(total) + ";" + std::to_string(init) + ";" + std::to_string(m("total")) +
+               ";" + std::to_string(m("first_fftw")) + ";" + std::to_string(m("first_trans")) + ";" +
+               std::to_string(m("second_fftw")) + ";" + std::to_string(m("second_trans")) + ";" +
+               std::to_string(m("plan")) + ";" + std::to_string(m("plan_flops")) + ";\n";
+    };
+    const std::string runtime_file_path = "runtimes/runtimes_hpx_shared_loop.txt";
+    create_parent_dir(runtime_file_path);
+    {
+        std::ofstream runtime_file(runtime_file_path, std::ios_base::app);
+        if (print_header) runtime_file << header;
+        runtime_file << line();
+    }
+    const std::string plan_file_path = "plans/plan_hpx_shared_loop.txt";
+    create_parent_dir(plan_file_path);
+    {
+        std::ofstream plan_info_file(plan_file_path, std::ios_base::app);
+        plan_info_file << header << line();
+    }
+    fft_computer.write_plans_to_file(plan_file_path);
+    return 0;
+}

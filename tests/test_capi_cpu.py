@@ -1,0 +1,150 @@
+"""C ABI and host-side surface, without a GPU.
+
+* libhpxfft_hip.so loads and exports every function include/hpxfft_hip.h declares;
+* plan-flag strings map as adapter_fftw.hpp:22-44 (invalid -> INVALID_ARG,
+  "Invalid FFTW plan flag string");
+* argument, size and layout errors come back as status codes (no exception
+  crosses the ABI), and with no HIP device plan creation returns NO_DEVICE;
+* the C++ programs built on the drop-in class report those errors cleanly.
+No compute call is made.
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import REPO
+
+import hpxfft_amd
+from hpxfft_amd import _lib
+
+HEADER = os.path.join(REPO, "include", "hpxfft_hip.h")
+BIN = os.path.join(REPO, "hpx-fft_amd", "bin")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(hpxfft_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = hpxfft_amd.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.EXPORTS)
+
+
+def test_version_and_last_error():
+    L = hpxfft_amd.lib()
+    assert b"gfx950" in L.hpxfft_version()
+    assert isinstance(hpxfft_amd.last_error(), str)
+
+
+@pytest.mark.parametrize("flag,effort", [("estimate", 0), ("measure", 1), ("patient", 2), ("exhaustive", 3)])
+def test_plan_flags(flag, effort):
+    assert hpxfft_amd.plan_flag_to_effort(flag) == effort
+
+
+@pytest.mark.parametrize("flag", ["", "Estimate", "fastest", "estimate "])
+def test_invalid_plan_flag(flag):
+    with pytest.raises(ValueError, match="Invalid FFTW plan flag string"):
+        hpxfft_amd.plan_flag_to_effort(flag)
+
+
+def plan_status(nx, ny, batch=1, ld_in=None, dist_in=None, ld_out=None, dist_out=None, prec=0, effort=0):
+    L = hpxfft_amd.lib()
+    ncy = ny // 2 + 1
+    ld_in = 2 * ncy if ld_in is None else ld_in
+    ld_out = ncy if ld_out is None else ld_out
+    dist_in = nx * ld_in if dist_in is None else dist_in
+    dist_out = nx * ld_out if dist_out is None else dist_out
+    h = ctypes.c_void_p()
+    st = L.hpxfft_plan_r2c_2d_many(ctypes.byref(h), nx, ny, batch, ld_in, dist_in, ld_out, dist_out, prec, effort,
+                                   None)
+    if h.value:
+        L.hpxfft_plan_destroy(h)
+    return st
+
+
+@pytest.mark.parametrize("nx,ny", [(3, 16), (16, 14), (16, 1), (0, 16), (16, 0), (1 << 17, 4), (4, 1 << 16)])
+def test_unsupported_sizes(nx, ny):
+    assert plan_status(nx, ny) == _lib.UNSUPPORTED_SIZE
+    assert hpxfft_amd.last_error()
+
+
+def test_bad_arguments():
+    L = hpxfft_amd.lib()
+    assert plan_status(16, 16, batch=0) == _lib.INVALID_ARG
+    assert plan_status(16, 16, prec=7) == _lib.INVALID_ARG
+    assert plan_status(16, 16, effort=9) == _lib.INVALID_ARG
+    assert plan_status(16, 16, ld_in=8) == _lib.INVALID_ARG          # ld_in < Ny
+    assert plan_status(16, 16, ld_out=4) == _lib.INVALID_ARG         # ld_out < Ny/2+1
+    assert plan_status(16, 16, ld_in=17) == _lib.INVALID_ARG         # odd pitch
+    assert plan_status(16, 16, batch=2, dist_in=8) == _lib.INVALID_ARG
+    assert L.hpxfft_plan_r2c_2d_many(None, 16, 16, 1, 18, 288, 9, 144, 0, 0, None) == _lib.INVALID_ARG
+    assert L.hpxfft_exec_r2c_2d(None, None, None) == _lib.INVALID_ARG
+    assert L.hpxfft_exec_r2c_2d_adjoint(None, None, None) == _lib.INVALID_ARG
+    assert L.hpxfft_exec_r2c_2d_host(None, None) == _lib.INVALID_ARG
+    assert L.hpxfft_exec_r2c_2d_scaled(None, None, None, None, None) == _lib.INVALID_ARG
+    v = ctypes.c_double()
+    assert L.hpxfft_get_measurement(None, b"total", ctypes.byref(v)) == _lib.INVALID_ARG
+    assert L.hpxfft_write_plan(None, b"/tmp/x") == _lib.INVALID_ARG
+    assert L.hpxfft_plan_destroy(None) == _lib.OK
+    eff = ctypes.c_int()
+    assert L.hpxfft_plan_flag_from_string(None, ctypes.byref(eff)) == _lib.INVALID_ARG
+
+
+def test_no_device_is_reported_cleanly():
+    st = plan_status(64, 64)
+    if st == _lib.OK:
+        pytest.skip("a HIP device is present")
+    assert st == _lib.NO_DEVICE
+    assert "no HIP device" in hpxfft_amd.last_error()
+    with pytest.raises(hpxfft_amd.HpxfftError):
+        hpxfft_amd.Plan(64, 64)
+
+
+def test_python_loop_surface_errors():
+    loop = hpxfft_amd.Loop()
+    import numpy as np
+
+    with pytest.raises(ValueError, match="Invalid FFTW plan flag string"):
+        loop.initialize(np.zeros((4, 6)), "bogus")
+    with pytest.raises(RuntimeError):
+        loop.fft_2d_r2c_par()
+    assert loop.get_measurement("anything") == 0.0
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "nope.so"))
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _lib.lib()
+
+
+def test_cpp_vector_2d():
+    r = subprocess.run([os.path.join(BIN, "test_vector_2d")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ALL PASSED" in r.stdout
+
+
+def test_cpp_harness_argument_errors(tmp_path):
+    exe = os.path.join(BIN, "shared_loop_2d_gpu")
+    r = subprocess.run([exe, "--plan=bogus", "--nx", "4", "--ny", "4"], capture_output=True, text=True, timeout=60,
+                       cwd=tmp_path)
+    assert r.returncode == 2 and "Invalid FFTW plan flag string" in r.stderr
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60, cwd=tmp_path)   # default 8 x 14
+    assert r.returncode == 2 and "powers of two" in r.stderr
+
+
+def test_cpp_known_answer_without_device():
+    st = plan_status(4, 4)
+    if st == _lib.OK:
+        pytest.skip("a HIP device is present (covered by the GPU test)")
+    r = subprocess.run([os.path.join(BIN, "test_shared_loop_gpu")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 3 and "no HIP device" in r.stderr
