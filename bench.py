@@ -85,6 +85,131 @@ def time_gpu(nx: int, ny: int, precision: str, steps: int, warmup: int, seed: in
             "col_s": sorted(cols)[len(cols) // 2], "out": out, "x": x, "plan": desc}
 
 
+def _sm64(seed: int, k: int) -> int:
+    """SplitMix64 output k of a generator seeded with `seed` (counter form)."""
+    m = (1 << 64) - 1
+    z = (seed + (k + 1) * 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def _events_ms(fn, reps: int, warmup: int = 2) -> float:
+    import torch
+    for _ in range(warmup):
+        fn()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(s)
+    for _ in range(reps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def bench_batched(count: int = 512, seed: int = 4, reps: int = 5) -> dict:
+    """512 uneven FP32 grids, sides 2^(6 + k), k = SplitMix64 output mod 6."""
+    import torch
+    import hpxfft_amd
+
+    sides = [64 << (_sm64(seed, 2 * i) % 6) for i in range(count)], [64 << (_sm64(seed, 2 * i + 1) % 6) for i in
+                                                                     range(count)]
+    nx, ny = sides
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    xs = []
+    for a, b in zip(nx, ny):
+        x = torch.empty((a, 2 * (b // 2 + 1)), dtype=torch.float32, device="cuda")
+        x[:, :b].uniform_(-1.0, 1.0, generator=g)
+        x[:, b:].zero_()
+        xs.append(x)
+    ptrs = [x.data_ptr() for x in xs]
+    stream = torch.cuda.current_stream()
+    tuned = hpxfft_amd.BatchedPlan(nx, ny, effort="measure", stream=stream)
+    default = hpxfft_amd.BatchedPlan(nx, ny, effort="estimate", stream=stream)
+    t_tuned = _events_ms(lambda: tuned.exec(ptrs), reps)
+    t_default = _events_ms(lambda: default.exec(ptrs), reps)
+    singles = [hpxfft_amd.Plan(a, b, precision="f32", stream=stream) for a, b in zip(nx, ny)]
+
+    def per_grid():
+        for p, x in zip(singles, xs):
+            p.exec(x.data_ptr(), x.data_ptr())
+
+    t_single = _events_ms(per_grid, reps)
+    alg = sum(a * b * 4 + a * (b // 2 + 1) * 8 for a, b in zip(nx, ny))   # 1-pass compulsory model
+    desc = tuned.describe()
+    for p in singles:
+        p.close()
+    tuned.close()
+    default.close()
+    return {"grids": count, "seed": seed, "alg_bytes_1pass": alg, "ms_autotuned": round(t_tuned, 4),
+            "ms_default_tiles": round(t_default, 4), "ms_per_grid_launches": round(t_single, 4),
+            "speedup_vs_per_grid": round(t_single / t_tuned, 3), "GBps_1pass": round(alg / (t_tuned * 1e-3) / 1e9, 1),
+            "plan": desc.strip().splitlines()[1:3]}
+
+
+def bench_adjoint(batch: int = 16, n: int = 4096, reps: int = 5) -> dict:
+    """Differentiable workload: batch 16 x 2^12 x 2^12 FP32, forward and adjoint."""
+    import torch
+    import hpxfft_amd
+
+    g = torch.Generator(device="cuda").manual_seed(5)
+    x = torch.randn(batch, n, n, dtype=torch.float32, device="cuda", generator=g)
+    X = hpxfft_amd.rfft2(x)
+    t_f = _events_ms(lambda: hpxfft_amd.rfft2(x), reps)
+    t_a = _events_ms(lambda: hpxfft_amd.rfft2_adjoint(X, n), reps)
+    B = batch * alg_bytes(n, n, 4)["total"]
+    return {"config": f"{batch} x {n} x {n} FP32", "fwd_ms": round(t_f, 4), "adjoint_ms": round(t_a, 4),
+            "fwd_GBps": round(B / (t_f * 1e-3) / 1e9, 1), "adjoint_GBps": round(B / (t_a * 1e-3) / 1e9, 1)}
+
+
+def bench_mixed(n: int = 1 << 15, reps: int = 3) -> dict:
+    """2^15 x 2^15 N(0,1): FP32, FP16-storage and BF16-storage rel-L2 vs the FP64 path, and times."""
+    import torch
+    import hpxfft_amd
+
+    ld = 2 * (n // 2 + 1)
+    g = torch.Generator(device="cuda").manual_seed(3)
+    x64 = torch.empty((n, ld), dtype=torch.float64, device="cuda")
+    x64[:, :n].normal_(generator=g)
+    x64[:, n:].zero_()
+    out = {}
+    ref = torch.empty((n, n // 2 + 1), dtype=torch.complex128, device="cuda")
+    p64 = hpxfft_amd.Plan(n, n, ld_in=ld, precision="f64")
+    p64.exec(x64.data_ptr(), ref.data_ptr())
+    out["f64_ms"] = round(_events_ms(lambda: p64.exec(x64.data_ptr(), ref.data_ptr()), reps), 3)
+    p64.close()
+    nref = torch.linalg.vector_norm(ref)
+    x32 = x64.float()
+    y32 = torch.empty((n, n // 2 + 1), dtype=torch.complex64, device="cuda")
+    p32 = hpxfft_amd.Plan(n, n, ld_in=ld, precision="f32")
+    t32 = _events_ms(lambda: p32.exec(x32.data_ptr(), y32.data_ptr()), reps)
+    p32.close()
+    err = (torch.linalg.vector_norm(y32.to(torch.complex128) - ref) / nref).item()
+    out["f32"] = {"ms": round(t32, 3), "rel_l2_vs_f64": err,
+                  "GBps": round(alg_bytes(n, n, 4)["total"] / (t32 * 1e-3) / 1e9, 1)}
+    del x32, y32
+    for name, dt, prec in (("f16", torch.float16, "f16"), ("bf16", torch.bfloat16, "bf16")):
+        x16 = x64.to(dt)
+        buf = torch.empty_like(x16)
+        col_exp = torch.empty(n // 2 + 1, dtype=torch.int32, device="cuda")
+        p = hpxfft_amd.Plan(n, n, ld_in=ld, ld_out=ld // 2, precision=prec)
+        # the copy alone first: the timed loop must be the last writer of buf
+        tc = _events_ms(lambda: buf.copy_(x16), reps)
+        t = _events_ms(lambda: (buf.copy_(x16), p.exec_scaled(buf.data_ptr(), buf.data_ptr(), None,
+                                                              col_exp.data_ptr())), reps)
+        p.close()
+        Xs = buf.view(n, ld // 2, 2).float()
+        X = torch.complex(Xs[..., 0], Xs[..., 1]).to(torch.complex128) * torch.exp2(-col_exp.double())[None, :]
+        e = (torch.linalg.vector_norm(X - ref) / nref).item()
+        out[name] = {"ms": round(t - tc, 3), "rel_l2_vs_f64": e,
+                     "GBps": round(alg_bytes(n, n, 2)["total"] / ((t - tc) * 1e-3) / 1e9, 1),
+                     "scales": "per input row after the row pass, per output column after the column pass"}
+        del x16, buf, Xs, X
+    return out
+
+
 def cpu_baseline(nx: int, ny: int, budget_s: float = 20.0):
     """The oracle's restatement of shared::loop::fft_2d_r2c_par on host cores."""
     import oracle
@@ -118,6 +243,7 @@ def main() -> int:
     ap.add_argument("--ny", type=int, default=1 << 14)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the batched / adjoint / 2^15 mixed-precision legs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the 2-D r2c path runs on a single MI355X (no multi-GPU path is built)",
@@ -184,6 +310,14 @@ def main() -> int:
                          "rel_l2_vs_f64": err,
                          "gflops": round(nominal_flops(nx, ny) / (r32["ms_per_step"] * 1e-3) / 1e9, 1),
                          "phases_s": {"row_r2c": r32["row_s"], "column_c2c": r32["col_s"]}}
+    if not args.no_extra:
+        torch.cuda.empty_cache()
+        result["batched_uneven"] = bench_batched()
+        torch.cuda.empty_cache()
+        result["adjoint"] = bench_adjoint()
+        torch.cuda.empty_cache()
+        result["mixed_2e15"] = bench_mixed()
+        torch.cuda.empty_cache()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(nx, ny)
     print(json.dumps(result), flush=True)

@@ -30,7 +30,7 @@ HIP_FLAGS = [
     "-I", INCLUDE,
 ]
 
-SOURCES = ["inst_f64.hip", "inst_f32.hip", "inst_f16.hip", "inst_bf16.hip", "capi.hip"]
+SOURCES = ["inst_f64.hip", "inst_f32.hip", "inst_f16.hip", "inst_bf16.hip", "inst_batched.hip", "capi.hip"]
 CPP_PROGRAMS = {
     # binary name : source under cpp/
     "shared_loop_2d_gpu": "shared_loop_2d_gpu.cpp",

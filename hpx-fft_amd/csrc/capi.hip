@@ -7,6 +7,7 @@
 // trans_values_vec_, which the fused column pass does not need at full
 // size), twiddle tables, timing events and the measurement map.
 #include "launch.hpp"
+#include "batched.hpp"
 #include "../../include/hpxfft_hip.h"
 
 #include <chrono>
@@ -199,7 +200,15 @@ struct hpxfft_plan_s
     // batched (uneven) plans
     bool batched = false;
     std::vector<size_t> bnx, bny;
-    std::vector<hpxfft_plan> sub;   // per-grid plans
+    std::vector<hpxfft_plan> sub;   // unused (kept empty)
+    int bcfg[2] = {0, 0};                                   // chosen configuration per phase
+    void *bgrids = nullptr;                                 // device grid descriptors
+    void *btasks[2][hpxfft_host::BATCH_NCFG] = {};          // device task tables
+    int bntasks[2][hpxfft_host::BATCH_NCFG] = {};
+    int bblocks[2][hpxfft_host::BATCH_NCFG] = {};
+    double btune_ms[2][hpxfft_host::BATCH_NCFG] = {};
+    unsigned *bticket = nullptr;                            // two work-queue counters
+    void *bstage = nullptr;                                 // pinned staging of the grid descriptors
 };
 
 namespace {
@@ -216,9 +225,23 @@ std::string describe(const hpxfft_plan_s *p)
     std::ostringstream os;
     if (p->batched)
     {
-        os << "hpxfft-hip batched plan: " << p->sub.size() << " grids, precision " << prec_name(p->prec) << "\n";
-        for (size_t i = 0; i < p->sub.size(); ++i)
-            os << "  grid " << i << ": " << p->bnx[i] << " x " << p->bny[i] << "\n";
+        os << "hpxfft-hip batched plan (gfx950): " << p->bnx.size() << " grids, precision " << prec_name(p->prec)
+           << ", effort " << p->effort << "\n";
+        for (int ph = 0; ph < 2; ++ph)
+        {
+            os << (ph == 0 ? "  row phase" : "  column phase") << ": persistent launch, configuration " << p->bcfg[ph]
+               << " (" << batched_pts(ph, p->bcfg[ph]) << " points per task, " << p->bntasks[ph][p->bcfg[ph]]
+               << " tasks, " << p->bblocks[ph][p->bcfg[ph]] << " workgroups)";
+            if (p->effort > 0)
+            {
+                os << "; autotuned ms:";
+                for (int c = 0; c < BATCH_NCFG; ++c) os << " cfg" << c << "=" << p->btune_ms[ph][c];
+            }
+            os << "\n";
+        }
+        size_t pts = 0;
+        for (size_t i = 0; i < p->bnx.size(); ++i) pts += p->bnx[i] * p->bny[i];
+        os << "  total real points " << pts << "\n";
         return os.str();
     }
     const geom rg = row_geom(p->prec, p->lgL);
@@ -420,6 +443,11 @@ void free_plan(hpxfft_plan_s *p)
     (void)hipFree(p->col_exp_own);
     (void)hipFree(p->adj_work);
     (void)hipFree(p->host_buf);
+    (void)hipFree(p->bgrids);
+    for (auto &ph : p->btasks)
+        for (auto &t : ph) (void)hipFree(t);
+    (void)hipFree(p->bticket);
+    if (p->bstage) (void)hipHostFree(p->bstage);
     delete p;
 }
 
@@ -625,6 +653,8 @@ int hpxfft_get_measurement(hpxfft_plan plan, const char *key, double *value)
             device_guard g(plan->device);
             HIPCHK(hipEventSynchronize(plan->ev[2]));
             if (k == "total") v = elapsed(plan->ev[0], plan->ev[2]);
+            else if (k == "first_fftw") v = elapsed(plan->ev[0], plan->ev[1]);
+            else v = elapsed(plan->ev[1], plan->ev[2]);
         }
     }
     else
@@ -669,24 +699,219 @@ int hpxfft_plan_destroy(hpxfft_plan plan)
     return HPXFFT_OK;
 }
 
+}   // extern "C"
+
+namespace {
+
+int batched_run(hpxfft_plan_s *p, int cfg_row, int cfg_col)
+{
+    const int cfg[2] = {cfg_row, cfg_col};
+    for (int ph = 0; ph < 2; ++ph)
+    {
+        hipLaunchKernelGGL(hpxfft_dev::zero_u32_kernel<0>, dim3(1), dim3(64), 0, p->stream, p->bticket + ph, 1LL);
+        HIPCHK(hipGetLastError());
+        const int c = cfg[ph];
+        const int nb = std::min(p->bblocks[ph][c], std::max(1, p->bntasks[ph][c]));
+        HIPCHK(batched_launch(ph, c, p->bgrids, p->btasks[ph][c], p->bntasks[ph][c], p->bticket + ph, p->tabs.tw32,
+                              nb, p->stream));
+        if (ph == 0) HIPCHK(hipEventRecord(p->ev[1], p->stream));
+    }
+    return HPXFFT_OK;
+}
+
+int batched_upload_grids(hpxfft_plan_s *p, void *const *d_grids)
+{
+    const size_t gb = batched_grid_desc_bytes();
+    char *st = reinterpret_cast<char *>(p->bstage);
+    for (size_t i = 0; i < p->bnx.size(); ++i)
+    {
+        if (!d_grids[i]) return fail(HPXFFT_INVALID_ARG, "null grid pointer " + std::to_string(i));
+        batched_fill_grid(st + i * gb, reinterpret_cast<float *>(d_grids[i]), (int)p->bnx[i], (int)p->bny[i]);
+    }
+    HIPCHK(hipMemcpyAsync(p->bgrids, p->bstage, gb * p->bnx.size(), hipMemcpyHostToDevice, p->stream));
+    return HPXFFT_OK;
+}
+
+// time every candidate configuration of both phases on scratch grids of the
+// plan's sizes and keep the fastest (the data values do not affect timing)
+int batched_autotune(hpxfft_plan_s *p)
+{
+    size_t total = 0;
+    std::vector<size_t> off(p->bnx.size());
+    for (size_t i = 0; i < p->bnx.size(); ++i)
+    {
+        off[i] = total;
+        total += p->bnx[i] * 2 * (p->bny[i] / 2 + 1);
+    }
+    float *buf = nullptr;
+    HIPCHK(hipMalloc(&buf, total * sizeof(float)));
+    std::vector<void *> ptrs(p->bnx.size());
+    for (size_t i = 0; i < ptrs.size(); ++i) ptrs[i] = buf + off[i];
+    int rc = batched_upload_grids(p, ptrs.data());
+    hipEvent_t a = nullptr, b = nullptr;
+    if (!rc && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess)
+    {
+        for (int ph = 0; ph < 2 && !rc; ++ph)
+        {
+            double best = 1e30;
+            for (int c = 0; c < BATCH_NCFG && !rc; ++c)
+            {
+                const int nb = std::min(p->bblocks[ph][c], std::max(1, p->bntasks[ph][c]));
+                float ms = 0.0f, acc = 0.0f;
+                for (int rep = 0; rep < 5 && !rc; ++rep)
+                {
+                    hipLaunchKernelGGL(hpxfft_dev::zero_u32_kernel<0>, dim3(1), dim3(64), 0, p->stream,
+                                       p->bticket + ph, 1LL);
+                    (void)hipEventRecord(a, p->stream);
+                    if (batched_launch(ph, c, p->bgrids, p->btasks[ph][c], p->bntasks[ph][c], p->bticket + ph,
+                                       p->tabs.tw32, nb, p->stream) != hipSuccess)
+                        rc = fail(HPXFFT_HIP_ERROR, "batched autotune launch failed");
+                    (void)hipEventRecord(b, p->stream);
+                    (void)hipEventSynchronize(b);
+                    (void)hipEventElapsedTime(&ms, a, b);
+                    if (rep >= 2) acc += ms;   // two warm-up runs
+                }
+                p->btune_ms[ph][c] = acc / 3.0;
+                if (p->btune_ms[ph][c] < best)
+                {
+                    best = p->btune_ms[ph][c];
+                    p->bcfg[ph] = c;
+                }
+            }
+        }
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipFree(buf);
+    return rc;
+}
+
+int plan_batched(hpxfft_plan_s *p)
+{
+    for (size_t i = 0; i < p->bnx.size(); ++i)
+    {
+        const size_t x = p->bnx[i], y = p->bny[i];
+        if (!is_pow2(x) || !is_pow2(y) || x < (size_t)BATCH_MIN_SIDE || y < (size_t)BATCH_MIN_SIDE ||
+            x > (size_t)BATCH_MAX_SIDE || y > (size_t)BATCH_MAX_SIDE)
+            return fail(HPXFFT_UNSUPPORTED_SIZE, "batched grids need power-of-two sides in [64, 2048] (grid " +
+                                                     std::to_string(i) + ": " + std::to_string(x) + " x " +
+                                                     std::to_string(y) + ")");
+    }
+    if (int rc = check_device(p->device)) return rc;
+    if (int rc = get_tables(p->device, p->tabs)) return rc;
+    if (int rc = alloc_events(p)) return rc;
+    const size_t n = p->bnx.size();
+    HIPCHK(hipMalloc(&p->bgrids, batched_grid_desc_bytes() * n));
+    HIPCHK(hipHostMalloc(&p->bstage, batched_grid_desc_bytes() * n, hipHostMallocDefault));
+    HIPCHK(hipMalloc(&p->bticket, 2 * sizeof(unsigned)));
+    const size_t tb = batched_task_desc_bytes();
+    for (int ph = 0; ph < 2; ++ph)
+    {
+        for (int c = 0; c < BATCH_NCFG; ++c)
+        {
+            const int pts = batched_pts(ph, c);
+            std::vector<char> tasks;
+            int nt = 0;
+            for (size_t i = 0; i < n; ++i)
+            {
+                const int nx = (int)p->bnx[i], ny = (int)p->bny[i];
+                if (ph == 0)
+                {
+                    const int rows = std::max(1, pts / (ny / 2));
+                    for (int r = 0; r < nx; r += rows, ++nt)
+                    {
+                        tasks.resize((size_t)(nt + 1) * tb);
+                        batched_fill_task(tasks.data() + (size_t)nt * tb, (int)i, r);
+                    }
+                }
+                else
+                {
+                    const int w = std::max(1, pts / nx), ncols = ny / 2 + 1;
+                    for (int st = 0; st * w < ncols; ++st, ++nt)
+                    {
+                        tasks.resize((size_t)(nt + 1) * tb);
+                        batched_fill_task(tasks.data() + (size_t)nt * tb, (int)i, st);
+                    }
+                }
+            }
+            p->bntasks[ph][c] = nt;
+            HIPCHK(hipMalloc(&p->btasks[ph][c], tasks.size()));
+            HIPCHK(hipMemcpy(p->btasks[ph][c], tasks.data(), tasks.size(), hipMemcpyHostToDevice));
+            p->bblocks[ph][c] = batched_resident_blocks(ph, c);
+        }
+    }
+    double pts = 0.0, fl = 0.0;
+    for (size_t i = 0; i < n; ++i)
+    {
+        const double x = (double)p->bnx[i], y = (double)p->bny[i];
+        pts += x * y;
+        fl += x * 2.5 * y * std::log2(y) + (y / 2 + 1) * 5.0 * x * std::log2(x);
+    }
+    p->plan_flops = fl;
+    if (p->effort >= HPXFFT_MEASURE)
+    {
+        if (int rc = batched_autotune(p)) return rc;
+    }
+    return HPXFFT_OK;
+}
+
+}   // namespace
+
+extern "C" {
+
 int hpxfft_plan_r2c_2d_batched(hpxfft_plan *plan, size_t count, const size_t *nx, const size_t *ny, int precision,
                                int effort, void *stream)
 {
-    (void)plan;
-    (void)count;
-    (void)nx;
-    (void)ny;
-    (void)precision;
-    (void)effort;
-    (void)stream;
-    return fail(HPXFFT_INVALID_ARG, "batched plans are not built yet");
+    try
+    {
+        if (!plan) return fail(HPXFFT_INVALID_ARG, "null plan pointer");
+        *plan = nullptr;
+        if (count == 0 || !nx || !ny) return fail(HPXFFT_INVALID_ARG, "need count >= 1 and size arrays");
+        if (precision != HPXFFT_F32) return fail(HPXFFT_INVALID_ARG, "batched plans are FP32");
+        if (effort < HPXFFT_ESTIMATE || effort > HPXFFT_EXHAUSTIVE) return fail(HPXFFT_INVALID_ARG, "unknown effort");
+        const auto t0 = std::chrono::steady_clock::now();
+        auto *p = new hpxfft_plan_s();
+        p->batched = true;
+        p->prec = precision;
+        p->effort = effort;
+        p->stream = reinterpret_cast<hipStream_t>(stream);
+        p->bnx.assign(nx, nx + count);
+        p->bny.assign(ny, ny + count);
+        if (int rc = plan_batched(p))
+        {
+            std::string msg = g_last_error;
+            free_plan(p);
+            g_last_error = msg;
+            return rc;
+        }
+        p->plan_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        *plan = p;
+        return HPXFFT_OK;
+    }
+    catch (const std::bad_alloc &)
+    {
+        return fail(HPXFFT_OUT_OF_MEMORY, "host allocation failed");
+    }
+    catch (const std::exception &e)
+    {
+        return fail(HPXFFT_HIP_ERROR, e.what());
+    }
 }
 
 int hpxfft_exec_r2c_2d_batched(hpxfft_plan plan, void *const *d_grids)
 {
-    (void)plan;
-    (void)d_grids;
-    return fail(HPXFFT_INVALID_ARG, "batched plans are not built yet");
+    if (!plan || !plan->batched) return fail(HPXFFT_INVALID_ARG, "not a batched plan");
+    if (!d_grids) return fail(HPXFFT_INVALID_ARG, "null grid pointer array");
+    device_guard g(plan->device);
+    // the pinned descriptor staging buffer may still be read by a previous upload
+    HIPCHK(hipStreamSynchronize(plan->stream));
+    if (int rc = batched_upload_grids(plan, d_grids)) return rc;
+    HIPCHK(hipEventRecord(plan->ev[0], plan->stream));
+    if (int rc = batched_run(plan, plan->bcfg[0], plan->bcfg[1])) return rc;
+    HIPCHK(hipEventRecord(plan->ev[2], plan->stream));
+    plan->ran = true;
+    return HPXFFT_OK;
 }
 
 }   // extern "C"

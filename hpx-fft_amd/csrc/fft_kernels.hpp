@@ -102,28 +102,22 @@ __device__ __forceinline__ int pick_exp(float m)
 }
 
 // ---------------------------------------------------------------------------
-template <typename T, typename SI, typename SO, int LG>
-__global__ void __launch_bounds__((cfg<T, SI>::row_NT(LG)))
-row_r2c_kernel(RowArgs a)
+// NSEQ rows [row0, row0+NSEQ) of grid b; the body is shared by the row
+// kernel and the batched (uneven grids) row-phase kernel.
+template <typename T, typename SI, typename SO, int L, int E, int NSEQ, bool SPLIT>
+__device__ __forceinline__ void row_r2c_body(const RowArgs &a, long long b, int row0,
+                                             typename Engine<T, L, E, NSEQ, false, SPLIT>::lds_t *lds,
+                                             unsigned *smax)
 {
-    constexpr int L = 1 << LG;
-    using C = cfg<T, SI>;
-    constexpr int E = C::row_E(LG), NSEQ = C::row_NSEQ(LG);
-    constexpr bool SPLIT = C::row_SPLIT(LG);
     using EN = Engine<T, L, E, NSEQ, false, SPLIT>;
     constexpr int NT = EN::NT;
     constexpr int HK = L / 2 + 1;                        // r2c pair tasks per row
     constexpr int NTASK = (NSEQ * HK + NT - 1) / NT;
 
-    __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
-    __shared__ unsigned smax[is16<SO>() ? NSEQ : 1];
-
     const int t = threadIdx.x;
-    const long long b = blockIdx.y;
     const SI *in = reinterpret_cast<const SI *>(a.in) + b * a.dist_in;
     SO *out = reinterpret_cast<SO *>(a.out) + b * a.dist_out;
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(a.tw);
-    const int row0 = blockIdx.x * NSEQ;
 
     cpx<T> v[E];
 #pragma unroll
@@ -250,27 +244,33 @@ row_r2c_kernel(RowArgs a)
     }
 }
 
+
+template <typename T, typename SI, typename SO, int LG>
+__global__ void __launch_bounds__((cfg<T, SI>::row_NT(LG)))
+row_r2c_kernel(RowArgs a)
+{
+    constexpr int L = 1 << LG;
+    using C = cfg<T, SI>;
+    constexpr int E = C::row_E(LG), NSEQ = C::row_NSEQ(LG);
+    constexpr bool SPLIT = C::row_SPLIT(LG);
+    using EN = Engine<T, L, E, NSEQ, false, SPLIT>;
+    __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
+    __shared__ unsigned smax[is16<SO>() ? NSEQ : 1];
+    row_r2c_body<T, SI, SO, L, E, NSEQ, SPLIT>(a, blockIdx.y, blockIdx.x * NSEQ, lds, smax);
+}
+
 // ---------------------------------------------------------------------------
 // column tile kernel.  blockIdx.x = strip of W columns inside the launch's
 // chunk, blockIdx.y = m1 (MODE 1) / k2 (MODE 2) / 0, blockIdx.z = grid.
-template <typename T, typename SI, typename SO, typename SC, int LG, int MODE, bool CONJ>
-__global__ void __launch_bounds__((cfg<T, SC>::template nt<MODE>(LG)))
-col_kernel(ColArgs a)
+// one tile: strip of W columns (x all L elements), sub = m1 / k2 / 0, grid b.
+// Shared by the column kernel and the batched column-phase kernel.
+template <typename T, typename SI, typename SO, int L, int E, int W, bool SPLIT, int MODE, bool CONJ>
+__device__ __forceinline__ void col_body(const ColArgs &a, long long b, int sub, int strip,
+                                         typename Engine<T, L, E, W, true, SPLIT>::lds_t *lds, unsigned *smax)
 {
-    constexpr int L = 1 << LG;
-    using C = cfg<T, SC>;   // SC = spectrum storage type: fixes the tile width W
-    constexpr int W = C::template width<MODE>(), E = C::template ept<MODE>(LG);
-    constexpr bool SPLIT = C::template split<MODE>(LG);
     using EN = Engine<T, L, E, W, true, SPLIT>;
-
-    __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
     constexpr bool OUT16 = is16<SO>() && MODE != 1;
-    __shared__ unsigned smax[(OUT16 || (MODE == 1 && is16<SI>())) ? W : 1];
-
     const int t = threadIdx.x;
-    const long long b = blockIdx.z;
-    const int sub = blockIdx.y;
-    const int strip = blockIdx.x;
     const int col0 = strip * W;
     const int nvalid = min(W, a.ncols - col0);
     const long long sbase = (long long)strip * a.P * a.Q * W;   // this strip's scratch block
@@ -368,6 +368,22 @@ col_kernel(ColArgs a)
             stor<SO>::st(out + 2 * idx, z);
         }
     }
+}
+
+
+template <typename T, typename SI, typename SO, typename SC, int LG, int MODE, bool CONJ>
+__global__ void __launch_bounds__((cfg<T, SC>::template nt<MODE>(LG)))
+col_kernel(ColArgs a)
+{
+    constexpr int L = 1 << LG;
+    using C = cfg<T, SC>;   // SC = spectrum storage type: fixes the tile width W
+    constexpr int W = C::template width<MODE>(), E = C::template ept<MODE>(LG);
+    constexpr bool SPLIT = C::template split<MODE>(LG);
+    using EN = Engine<T, L, E, W, true, SPLIT>;
+    __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
+    constexpr bool OUT16 = is16<SO>() && MODE != 1;
+    __shared__ unsigned smax[(OUT16 || (MODE == 1 && is16<SI>())) ? W : 1];
+    col_body<T, SI, SO, L, E, W, SPLIT, MODE, CONJ>(a, blockIdx.z, blockIdx.y, blockIdx.x, lds, smax);
 }
 
 // ---------------------------------------------------------------------------

@@ -6,9 +6,9 @@ reached through the C ABI of ``include/hpxfft_hip.h``; this package is the
 Python host layer (ctypes) and the torch binding.
 """
 from ._lib import HpxfftError, lib, last_error, LIB_PATH  # noqa: F401
-from .plan import Plan, Loop, Sync, Opt, Naive, plan_flag_to_effort  # noqa: F401
+from .plan import Plan, BatchedPlan, Loop, Sync, Opt, Naive, plan_flag_to_effort  # noqa: F401
 
-__all__ = ["HpxfftError", "lib", "last_error", "LIB_PATH", "Plan", "Loop", "Sync", "Opt", "Naive",
+__all__ = ["HpxfftError", "lib", "last_error", "LIB_PATH", "Plan", "BatchedPlan", "Loop", "Sync", "Opt", "Naive",
            "plan_flag_to_effort"]
 
 

@@ -104,6 +104,37 @@ class Plan:
             pass
 
 
+class BatchedPlan:
+    """Many independent in-place FP32 grids of uneven sizes (sides 64..2048),
+    transformed by two persistent launches (row phase, column phase).
+
+    ``effort`` "measure" (or higher) times every candidate task shape on the
+    plan's sizes and keeps the fastest per phase (the autotuner).
+    """
+
+    def __init__(self, nx, ny, *, effort: int | str = "estimate", stream=None):
+        assert len(nx) == len(ny) and len(nx) > 0
+        self.nx, self.ny = [int(v) for v in nx], [int(v) for v in ny]
+        n = len(self.nx)
+        eff = plan_flag_to_effort(effort) if isinstance(effort, str) else int(effort)
+        arr = ctypes.c_size_t * n
+        h = ctypes.c_void_p()
+        check(lib().hpxfft_plan_r2c_2d_batched(ctypes.byref(h), n, arr(*self.nx), arr(*self.ny), _lib.F32, eff,
+                                               _stream_handle(stream)))
+        self._h = h
+        self._ptrs = (ctypes.c_void_p * n)()
+
+    def exec(self, d_grids) -> None:
+        for i, p in enumerate(d_grids):
+            self._ptrs[i] = p
+        check(lib().hpxfft_exec_r2c_2d_batched(self._h, self._ptrs))
+
+    measurement = Plan.measurement
+    describe = Plan.describe
+    close = Plan.close
+    __del__ = Plan.__del__
+
+
 class Loop:
     """hpxfft::shared::loop on the GPU, host buffers in and out."""
 

@@ -148,3 +148,22 @@ def test_cpp_known_answer_without_device():
         pytest.skip("a HIP device is present (covered by the GPU test)")
     r = subprocess.run([os.path.join(BIN, "test_shared_loop_gpu")], capture_output=True, text=True, timeout=60)
     assert r.returncode == 3 and "no HIP device" in r.stderr
+
+
+def test_batched_plan_argument_errors():
+    L = hpxfft_amd.lib()
+    arr = ctypes.c_size_t * 3
+    h = ctypes.c_void_p()
+    # side outside [64, 2048] or not a power of two -> UNSUPPORTED_SIZE (checked before any device call)
+    assert L.hpxfft_plan_r2c_2d_batched(ctypes.byref(h), 3, arr(64, 128, 32), arr(64, 64, 64), 1, 0,
+                                        None) == _lib.UNSUPPORTED_SIZE
+    assert L.hpxfft_plan_r2c_2d_batched(ctypes.byref(h), 3, arr(64, 128, 4096), arr(64, 64, 64), 1, 0,
+                                        None) == _lib.UNSUPPORTED_SIZE
+    assert L.hpxfft_plan_r2c_2d_batched(ctypes.byref(h), 3, arr(64, 96, 64), arr(64, 64, 64), 1, 0,
+                                        None) == _lib.UNSUPPORTED_SIZE
+    # FP32 only; count >= 1
+    assert L.hpxfft_plan_r2c_2d_batched(ctypes.byref(h), 3, arr(64, 64, 64), arr(64, 64, 64), 0, 0,
+                                        None) == _lib.INVALID_ARG
+    assert L.hpxfft_plan_r2c_2d_batched(ctypes.byref(h), 0, arr(64, 64, 64), arr(64, 64, 64), 1, 0,
+                                        None) == _lib.INVALID_ARG
+    assert L.hpxfft_exec_r2c_2d_batched(None, None) == _lib.INVALID_ARG
