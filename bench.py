@@ -10,7 +10,14 @@ value = algorithmic bytes / time, 2-pass model (transposes not credited):
     B = [Nx*Ny*s + H] + [2H],  H = Nx*(Ny/2+1)*2*s,  s = bytes per real.
 
 Prints ONE JSON line.  Flags: --steps K --warmup W --gpus N (N must be 1)
---no-cpu (skip the CPU baseline leg) --nx/--ny (other sizes).
+--no-cpu (skip the CPU baseline leg) --no-f32 --no-extra (skip the other
+legs) --nx/--ny (other sizes) --copy-probe BYTES (only a device copy, for
+calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE).
+
+roofline: the dominant launch (longest event-timed duration on the plan's
+stream); achieved = its algorithmic bytes / its average duration; traffic =
+its HBM bytes per launch from profiles/traffic.json (rocprofv3 counters
+scaled by the copy-kernel factor), null when no summary covers the workload.
 """
 from __future__ import annotations
 
@@ -74,15 +81,59 @@ def time_gpu(nx: int, ny: int, precision: str, steps: int, warmup: int, seed: in
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1) / steps
     # per-phase times from the plan's own HIP events (row pass, column pass)
-    rows, cols = [], []
+    keys = ("first_fftw", "second_fftw", "column_a", "column_b")
+    ph = {k: [] for k in keys}
     for _ in range(max(3, min(steps, 10))):
         plan.exec(x.data_ptr(), out.data_ptr())
-        rows.append(plan.measurement("first_fftw"))
-        cols.append(plan.measurement("second_fftw"))
+        for k in keys:
+            ph[k].append(plan.measurement(k))
+    med = {k: sorted(v)[len(v) // 2] for k, v in ph.items()}
     desc = plan.describe()
     plan.close()
-    return {"ms_per_step": ms, "wall_s": wall, "row_s": sorted(rows)[len(rows) // 2],
-            "col_s": sorted(cols)[len(cols) // 2], "out": out, "x": x, "plan": desc}
+    return {"ms_per_step": ms, "wall_s": wall, "row_s": med["first_fftw"], "col_s": med["second_fftw"],
+            "col_a_s": med["column_a"], "col_b_s": med["column_b"], "out": out, "x": x, "plan": desc}
+
+
+def launches(nx: int, ny: int, s: int, r: dict) -> dict:
+    """Per-launch algorithmic bytes and event-timed duration.  A four-step
+    column pass is two launches; each is credited half of the column pass's
+    2H (phase A reads the spectrum, phase B writes it; the scratch between
+    them is not algorithmic)."""
+    b = alg_bytes(nx, ny, s)
+    out = {"row_r2c": (b["row"], r["row_s"])}
+    if r["col_a_s"] > 0.0:
+        out["column_phase_a"] = (b["col"] // 2, r["col_a_s"])
+        out["column_phase_b"] = (b["col"] // 2, r["col_b_s"])
+    else:
+        out["column_c2c"] = (b["col"], r["col_s"])
+    return out
+
+
+def measured_traffic(workload: str) -> dict:
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary (profiles/traffic.json), already scaled by the copy-kernel factor."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(workload, {})
+    except (OSError, ValueError):
+        return {}
+
+
+def progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def copy_probe(nbytes: int, reps: int = 3) -> int:
+    """A plain device copy of known size (counter calibration under rocprofv3 --pmc)."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+    return 0
 
 
 def _sm64(seed: int, k: int) -> int:
@@ -244,6 +295,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched / adjoint / 2^15 mixed-precision legs")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (reads BYTES, writes BYTES) three times")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the 2-D r2c path runs on a single MI355X (no multi-GPU path is built)",
@@ -254,15 +307,20 @@ def main() -> int:
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         return 2
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     nx, ny = args.nx, args.ny
     r64 = time_gpu(nx, ny, "f64", args.steps, args.warmup, seed=2)
+    progress(f"f64 {nx}x{ny}: {r64['ms_per_step']:.4f} ms/step")
     b64 = alg_bytes(nx, ny, 8)
     gbs = b64["total"] / (r64["ms_per_step"] * 1e-3) / 1e9
-    # dominant kernel phase: row pass or column pass (whichever is longer)
-    phases = {"row_r2c": (b64["row"], r64["row_s"]), "column_c2c": (b64["col"], r64["col_s"])}
-    dom = max(phases, key=lambda k: phases[k][1])
-    dom_bytes, dom_s = phases[dom]
+    # dominant kernel: the launch with the longest event-timed duration on the plan's stream
+    per_launch = launches(nx, ny, 8, r64)
+    dom = max(per_launch, key=lambda k: per_launch[k][1])
+    dom_bytes, dom_s = per_launch[dom]
     ach = dom_bytes / dom_s / 1e9
+    workload = f"f64_{nx}x{ny}"
+    traffic = measured_traffic(workload).get(dom)
     result = {
         "metric": "2D r2c FFT GB/s & %HBM peak at 2^14^2, FP32 vs FP64 at stated L2 error",
         "value": round(gbs, 2),
@@ -281,13 +339,15 @@ def main() -> int:
         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
         "gflops": round(nominal_flops(nx, ny) / (r64["ms_per_step"] * 1e-3) / 1e9, 1),
         "roofline": {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                      "alg_bytes_per_launch": dom_bytes, "avg_launch_s": dom_s},
+        "launches_s": {k: v[1] for k, v in per_launch.items()},
         "phases_s": {"row_r2c": r64["row_s"], "column_c2c": r64["col_s"]},
     }
     result["fp64_vector_frac"] = round(result["gflops"] / (FP64_VEC_TFLOPS * 1e3), 4)
     if not args.no_f32:
         r32 = time_gpu(nx, ny, "f32", args.steps, args.warmup, seed=2)
+        progress(f"f32 {nx}x{ny}: {r32['ms_per_step']:.4f} ms/step")
         b32 = alg_bytes(nx, ny, 4)
         # FP32 error against the FP64 result of the same input (the FP64 path matches the oracle to 1e-12)
         x64 = r64["x"]
@@ -313,13 +373,17 @@ def main() -> int:
     if not args.no_extra:
         torch.cuda.empty_cache()
         result["batched_uneven"] = bench_batched()
+        progress("batched uneven grids done")
         torch.cuda.empty_cache()
         result["adjoint"] = bench_adjoint()
+        progress("adjoint done")
         torch.cuda.empty_cache()
         result["mixed_2e15"] = bench_mixed()
+        progress("2^15 mixed precision done")
         torch.cuda.empty_cache()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(nx, ny)
+        progress("cpu baseline done")
     print(json.dumps(result), flush=True)
     return 0
 

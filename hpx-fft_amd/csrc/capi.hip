@@ -192,8 +192,9 @@ struct hpxfft_plan_s
     void *host_buf = nullptr;     // drop-in host path: device copy of the vector_2d
     size_t host_bytes = 0;
     // timing
-    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // start, rows done, columns done, column phase A done
     bool ran = false;
+    bool col_split = false;      // ev[3] valid: last run was a single-grid four-step forward
     double h2d = 0.0, d2h = 0.0;
     double plan_seconds = 0.0, plan_flops = 0.0;
     std::map<std::string, double> extra;
@@ -284,7 +285,8 @@ int alloc_events(hpxfft_plan_s *p)
 
 // column pass on the spectrum `spec` (in place), or from `spec_in` to
 // `spec_out` for the adjoint; `conj` = inverse transform.
-int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj, int *col_exp)
+int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj, int *col_exp,
+                hipEvent_t mid = nullptr)
 {
     const size_t sb = storage_bytes(p->prec);
     const int W = p->four_step ? tile_geom(p->prec, p->lgQ).nseq : p->W;
@@ -335,6 +337,7 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         ab.out = p->scratch;
         ab.tw_L = tw_ptr(p, (size_t)Q);
         HIPCHK(col(p->prec, p->lgQ, 1, conj, ab, gx, (unsigned)P, 1, p->stream));
+        if (mid && p->batch == 1) HIPCHK(hipEventRecord(mid, p->stream));
         // phase B: scratch -> spectrum
         ab.in = p->scratch;
         ab.out = out_b;
@@ -371,9 +374,10 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     const unsigned gx = (unsigned)((p->nx + rg.nseq - 1) / rg.nseq);
     HIPCHK(row_r2c(p->prec, p->lgL, r, gx, (unsigned)p->batch, p->stream));
     HIPCHK(hipEventRecord(p->ev[1], p->stream));
-    if (int rc = run_columns(p, d_out, d_out, false, s16 ? col_exp : nullptr)) return rc;
+    if (int rc = run_columns(p, d_out, d_out, false, s16 ? col_exp : nullptr, p->ev[3])) return rc;
     HIPCHK(hipEventRecord(p->ev[2], p->stream));
     p->ran = true;
+    p->col_split = p->four_step && p->batch == 1;
     p->h2d = p->d2h = 0.0;
     return HPXFFT_OK;
 }
@@ -625,6 +629,7 @@ int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_rea
     HIPCHK(row_c2r(plan->prec, plan->lgL, r, gx, (unsigned)plan->batch, plan->stream));
     HIPCHK(hipEventRecord(plan->ev[2], plan->stream));
     plan->ran = true;
+    plan->col_split = false;
     return HPXFFT_OK;
 }
 
@@ -640,7 +645,7 @@ int hpxfft_get_measurement(hpxfft_plan plan, const char *key, double *value)
     else if (k == "d2h") v = plan->d2h;
     else if (k == "total" || k == "first_fftw" || k == "second_fftw")
     {
-        if (plan->ran && !plan->batched)
+        if (plan->ran)
         {
             device_guard g(plan->device);
             HIPCHK(hipEventSynchronize(plan->ev[2]));
@@ -648,13 +653,15 @@ int hpxfft_get_measurement(hpxfft_plan plan, const char *key, double *value)
             else if (k == "first_fftw") v = elapsed(plan->ev[0], plan->ev[1]);
             else v = elapsed(plan->ev[1], plan->ev[2]);
         }
-        else if (plan->ran && plan->batched)
+    }
+    else if (k == "column_a" || k == "column_b")
+    {
+        // the two launches of a four-step column pass (0 when the last run had no split)
+        if (plan->ran && plan->col_split)
         {
             device_guard g(plan->device);
             HIPCHK(hipEventSynchronize(plan->ev[2]));
-            if (k == "total") v = elapsed(plan->ev[0], plan->ev[2]);
-            else if (k == "first_fftw") v = elapsed(plan->ev[0], plan->ev[1]);
-            else v = elapsed(plan->ev[1], plan->ev[2]);
+            v = k == "column_a" ? elapsed(plan->ev[1], plan->ev[3]) : elapsed(plan->ev[3], plan->ev[2]);
         }
     }
     else
@@ -911,6 +918,7 @@ int hpxfft_exec_r2c_2d_batched(hpxfft_plan plan, void *const *d_grids)
     if (int rc = batched_run(plan, plan->bcfg[0], plan->bcfg[1])) return rc;
     HIPCHK(hipEventRecord(plan->ev[2], plan->stream));
     plan->ran = true;
+    plan->col_split = false;
     return HPXFFT_OK;
 }
 

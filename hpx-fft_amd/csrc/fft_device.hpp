@@ -19,6 +19,36 @@
 namespace hpxfft_dev {
 
 // ---------------------------------------------------------------------------
+// XCD-aware block order.  gfx950 dispatches workgroup i (x fastest, then y,
+// then z) to XCD i % 8, and each XCD has its own L2.  xcd_order(i, n) maps the
+// dispatch index to a logical index such that XCD x works through one
+// contiguous logical range, in order: neighbouring rows / column tiles then
+// run at the same time behind the same L2, so the 128-byte lines they share
+// (row pitches such as (Ny/2+1) complex are not multiples of 128 bytes) are
+// fetched once and written back whole.  A bijection on [0, n) whatever the
+// placement, so only locality depends on it.
+constexpr int NUM_XCD = 8;
+
+__device__ __forceinline__ int xcd_order(int i, int n)
+{
+    const int q = n / NUM_XCD, r = n % NUM_XCD;
+    const int x = i % NUM_XCD, k = i / NUM_XCD;
+    return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + k;
+}
+
+struct blk3 { int x, y, z; };
+
+// the launch's (blockIdx.x, .y, .z) in XCD-contiguous logical order
+__device__ __forceinline__ blk3 xcd_block()
+{
+    const int gx = (int)gridDim.x, gy = (int)gridDim.y;
+    const int n = gx * gy * (int)gridDim.z;
+    const int l = xcd_order((int)blockIdx.x + gx * ((int)blockIdx.y + gy * (int)blockIdx.z), n);
+    const int yz = l / gx;
+    return {l - yz * gx, yz % gy, yz / gy};
+}
+
+// ---------------------------------------------------------------------------
 // complex helpers
 template <typename T> struct cpx { T x, y; };
 

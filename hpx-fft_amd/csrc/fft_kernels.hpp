@@ -256,7 +256,8 @@ row_r2c_kernel(RowArgs a)
     using EN = Engine<T, L, E, NSEQ, false, SPLIT>;
     __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
     __shared__ unsigned smax[is16<SO>() ? NSEQ : 1];
-    row_r2c_body<T, SI, SO, L, E, NSEQ, SPLIT>(a, blockIdx.y, blockIdx.x * NSEQ, lds, smax);
+    const blk3 bk = xcd_block();
+    row_r2c_body<T, SI, SO, L, E, NSEQ, SPLIT>(a, bk.y, bk.x * NSEQ, lds, smax);
 }
 
 // ---------------------------------------------------------------------------
@@ -383,7 +384,8 @@ col_kernel(ColArgs a)
     __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
     constexpr bool OUT16 = is16<SO>() && MODE != 1;
     __shared__ unsigned smax[(OUT16 || (MODE == 1 && is16<SI>())) ? W : 1];
-    col_body<T, SI, SO, L, E, W, SPLIT, MODE, CONJ>(a, blockIdx.z, blockIdx.y, blockIdx.x, lds, smax);
+    const blk3 bk = xcd_block();
+    col_body<T, SI, SO, L, E, W, SPLIT, MODE, CONJ>(a, bk.z, bk.y, bk.x, lds, smax);
 }
 
 // ---------------------------------------------------------------------------
@@ -408,11 +410,12 @@ row_c2r_kernel(RowArgs a)
     __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
 
     const int t = threadIdx.x;
-    const long long b = blockIdx.y;
+    const blk3 bk = xcd_block();
+    const long long b = bk.y;
     const SI *in = reinterpret_cast<const SI *>(a.in) + b * a.dist_in;
     SO *out = reinterpret_cast<SO *>(a.out) + b * a.dist_out;
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(a.tw);
-    const int row0 = blockIdx.x * NSEQ;
+    const int row0 = bk.x * NSEQ;
 
     // pre-processing: pairs (k, L-k) -> conj Z[k], conj Z[L-k]
     cpx<T> Z0[NTASK], Z1[NTASK];

@@ -247,7 +247,7 @@ row_r2c_big_kernel(const T *__restrict__ in, T *__restrict__ out, long long ld_i
     constexpr int NT = B::NT;
     __shared__ T lds[B::SPLIT ? B::LP : 2 * B::LP];
     const int t = threadIdx.x;
-    const int g = blockIdx.x;
+    const int g = xcd_order((int)blockIdx.x, (int)gridDim.x);   // consecutive rows share an XCD
     const int b = g / nrows, row = g - b * nrows;
     const T *irow = in + (long long)b * dist_in + (long long)row * ld_in;
     cpx<T> v[16];

@@ -78,8 +78,9 @@ tile_kernel(ColArgs a)
 
     const int t = threadIdx.x;
     const int w = t % W, jj = t / W;   // column in the tile, butterfly index in the column
-    const int sub = blockIdx.y;        // m1 (MODE 1) or k2 (MODE 2)
-    const int strip = blockIdx.x;
+    const blk3 bk = xcd_block();       // neighbouring strips of one sub share an XCD
+    const int sub = bk.y;              // m1 (MODE 1) or k2 (MODE 2)
+    const int strip = bk.x;
     const int col0 = strip * W;
     const bool valid = w < a.ncols - col0;
     const long long sbase = (long long)strip * a.P * a.Q * W;

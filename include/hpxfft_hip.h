@@ -113,7 +113,9 @@ int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_rea
 /* Replaces hpxfft::shared::loop::get_measurement (loop.cpp:192): seconds of
  * the last exec for "total", "first_fftw", "first_trans", "second_fftw",
  * "second_trans" (the transposes are fused into the column pass and read
- * 0), "h2d", "d2h"; "plan" (planning seconds) and "plan_flops".  Unknown keys
+ * 0), "h2d", "d2h"; "plan" (planning seconds) and "plan_flops"; and, for a
+ * single grid whose column pass runs as two launches, "column_a" and
+ * "column_b" (seconds of each launch; 0 otherwise).  Unknown keys
  * give 0.0, as std::map::operator[] does in the reference.  Waits for the
  * last exec to finish. */
 int hpxfft_get_measurement(hpxfft_plan plan, const char *key, double *value);
