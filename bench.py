@@ -129,10 +129,10 @@ def copy_probe(nbytes: int, reps: int = 3) -> int:
     import torch
     src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
     dst = torch.empty_like(src)
-    for _ in range(reps):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+    dst.copy_(src)
+    ms = _events_ms(lambda: dst.copy_(src), reps, warmup=1)
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps, "ms": round(ms, 4),
+                      "GBps_read_plus_write": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}), flush=True)
     return 0
 
 

@@ -98,6 +98,32 @@ def test_autograd_vs_torch_fp64_cpu(gpu, nx, ny):
     assert err <= 1e-12, err
 
 
+@pytest.mark.parametrize("nx,ny", [(8, 16), (64, 128)])
+def test_full_spectrum_loss_with_hermitian_weights(gpu, nx, ny):
+    """The same loss written over the FULL spectrum, sum_k w(k)|X_k|^2 through
+    torch.fft.fft2 (FP64, CPU autograd), equals the half-spectrum loss with
+    Hermitian weights c(ky) = 1 at ky in {0, Ny/2} and 2 in between (w is even
+    in k).  Both the value and the RFFT2 gradient must agree with it."""
+    import torch
+    import hpxfft_amd
+    xc = torch.randn(nx, ny, dtype=torch.float64)
+    kx = torch.fft.fftfreq(nx, 1.0 / nx, dtype=torch.float64)
+    ky = torch.fft.fftfreq(ny, 1.0 / ny, dtype=torch.float64)
+    w_full = 1.0 / (1.0 + kx[:, None] ** 2 + ky[None, :] ** 2)
+    xr = xc.clone().requires_grad_(True)
+    lf = torch.sum(w_full * torch.abs(torch.fft.fft2(xr)) ** 2)
+    lf.backward()
+    c = torch.full((ny // 2 + 1,), 2.0, dtype=torch.float64)
+    c[0] = c[-1] = 1.0
+    wh = weights(nx, ny, gpu, torch.float64) * c.to(gpu)[None, :]
+    xg = xc.to(gpu).requires_grad_(True)
+    lh = loss_fn(hpxfft_amd.rfft2_autograd(xg), wh)
+    lh.backward()
+    assert abs(lh.item() - lf.item()) <= 1e-12 * abs(lf.item())
+    err = oracle.rel_l2(xg.grad.cpu().numpy(), xr.grad.numpy())
+    assert err <= 1e-12, err
+
+
 def directional_check(x32, h=1.0, ndir=3, seed=0):
     """<grad L(x32), d> from the FP32 adjoint vs FP64 central differences of L.
 
