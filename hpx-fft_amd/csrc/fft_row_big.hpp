@@ -10,9 +10,14 @@
 // base plus a compile-time stride (immediate offsets).  FP64 rows exchange
 // real and imaginary halves separately so the LDS footprint is 8 bytes per
 // point (two workgroups per CU at L = 8192).
+//
+// 16-bit storage (FP32 arithmetic): inputs are converted on load and scaled
+// by 2^-in_exp[row] when given; the row's outputs stay in registers until the
+// workgroup has their largest component, then are stored as X * 2^e with
+// e = pick_exp(max) written to out_exp[row].
 #pragma once
 
-#include "fft_device.hpp"
+#include "fft_kernels.hpp"
 
 namespace hpxfft_dev {
 
@@ -158,24 +163,42 @@ __device__ __forceinline__ void gather_fused(T *lds, const cpx<T> *v, cpx<T> (&a
     __syncthreads();
 }
 
-template <typename T, typename SO>
-__device__ __forceinline__ void r2c_pair(SO *orow, int L, int k, cpx<T> A, cpx<T> Bc, const cpx<T> *__restrict__ tw)
+// r2c split of one pair: A = Z[k], Bc = Z[L-k];
+// X[k] = Fe + w_{2L}^k Fo, X[L-k] = conj(Fe - w_{2L}^k Fo)
+template <typename T>
+__device__ __forceinline__ void r2c_val(cpx<T> A, cpx<T> Bc, cpx<T> w, cpx<T> &xk, cpx<T> &xlk)
 {
-    // X[k] = Fe + w_{2L}^k Fo, X[L-k] = conj(Fe - w_{2L}^k Fo)
     const cpx<T> Bk = conjc(Bc);
     const cpx<T> Fe = scal(A + Bk, T(0.5));
     const cpx<T> D = scal(A - Bk, T(0.5));
     const cpx<T> Fo = {D.y, -D.x};
-    const cpx<T> tt = cmul(tw[k], Fo);
-    stor<SO>::st(orow + 2 * k, Fe + tt);
-    stor<SO>::st(orow + 2 * (L - k), conjc(Fe - tt));
+    const cpx<T> tt = cmul(w, Fo);
+    xk = Fe + tt;
+    xlk = conjc(Fe - tt);
+}
+
+template <typename T, typename SO>
+__device__ __forceinline__ void r2c_pair(SO *orow, int L, int k, cpx<T> A, cpx<T> Bc, const cpx<T> *__restrict__ tw)
+{
+    cpx<T> x0, x1;
+    r2c_val(A, Bc, tw[k], x0, x1);
+    stor<SO>::st(orow + 2 * k, x0);
+    stor<SO>::st(orow + 2 * (L - k), x1);
+}
+
+template <typename T>
+__device__ __forceinline__ float cmax(cpx<T> a, cpx<T> b)
+{
+    return (float)fmax(fmax(fabs(a.x), fabs(a.y)), fmax(fabs(b.x), fabs(b.y)));
 }
 
 // Transform one row held in v[] (stage-0 inputs: v[r] = z[t + r NT]) and
-// store its Ny/2+1 outputs to orow.
-template <typename T, int LG>
+// store its Ny/2+1 outputs to orow (storage SO).  16-bit SO: smax is a
+// workgroup slot cleared before the first barrier, oexp the row's exponent.
+template <typename T, typename SO, int LG>
 __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__restrict__ tw,
-                                             const cpx<double> *__restrict__ tw64, T *orow, int t)
+                                             const cpx<double> *__restrict__ tw64, SO *orow, int t,
+                                             unsigned *smax, int *oexp)
 {
     constexpr int L = 1 << LG;
     using B = bigrow<T, L>;
@@ -193,67 +216,168 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
     gather_fused<T, L, false>(lds, v, a, c, s, t);
     if constexpr (B::SPLIT) gather_fused<T, L, true>(lds, v, a, c, s, t);
 
-#pragma unroll
-    for (int i = 0; i < B::NMAIN; ++i)
-    {
-        const int j = t + i * NT;
-        const int jp = NSL - j;   // == NSL for j == 0 (then c == a)
-        // final-stage twiddles w_L^{r b} = w_{2L}^{2 r b}
-#pragma unroll
-        for (int r = 1; r < RL; ++r)
-        {
-            a[i][r] = cmul(a[i][r], tw[2 * r * j]);
-            c[i][r] = cmul(c[i][r], tw[(2 * r * jp) & (2 * L - 1)]);
-        }
-        dftR<T, RL>(a[i]);
-        dftR<T, RL>(c[i]);
-        // a[q] = Z[j + q NSL], c[q] = Z[jp + q NSL]
-        if (j == 0)
-        {
-            const cpx<T> z0 = a[i][0];
-            stor<T>::st(orow, cpx<T>{z0.x + z0.y, T(0)});
-            stor<T>::st(orow + 2 * L, cpx<T>{z0.x - z0.y, T(0)});
-#pragma unroll
-            for (int q = 1; q < RL / 2; ++q) r2c_pair<T, T>(orow, L, q * NSL, a[i][q], a[i][RL - q], tw);
-            stor<T>::st(orow + 2 * (L / 2), conjc(a[i][RL / 2]));
-        }
-        else
-        {
-#pragma unroll
-            for (int q = 0; q < RL; ++q) r2c_pair<T, T>(orow, L, j + q * NSL, a[i][q], c[i][RL - 1 - q], tw);
-        }
-    }
-    if (t == HALF % NT)
+    const bool self = t == HALF % NT;
+    if (self)
     {
         // self-paired butterfly j = HALF: pairs (q, RL-1-q)
 #pragma unroll
         for (int r = 1; r < RL; ++r) s[r] = cmul(s[r], tw[2 * r * HALF]);
         dftR<T, RL>(s);
+    }
+
+    if constexpr (!is16<SO>())
+    {
+        // FP64 / FP32 storage: store as produced
 #pragma unroll
-        for (int q = 0; q < RL / 2; ++q) r2c_pair<T, T>(orow, L, HALF + q * NSL, s[q], s[RL - 1 - q], tw);
+        for (int i = 0; i < B::NMAIN; ++i)
+        {
+            const int j = t + i * NT;
+            const int jp = NSL - j;   // == NSL for j == 0 (then c == a)
+            // final-stage twiddles w_L^{r b} = w_{2L}^{2 r b}
+#pragma unroll
+            for (int r = 1; r < RL; ++r)
+            {
+                a[i][r] = cmul(a[i][r], tw[2 * r * j]);
+                c[i][r] = cmul(c[i][r], tw[(2 * r * jp) & (2 * L - 1)]);
+            }
+            dftR<T, RL>(a[i]);
+            dftR<T, RL>(c[i]);
+            // a[q] = Z[j + q NSL], c[q] = Z[jp + q NSL]
+            if (j == 0)
+            {
+                const cpx<T> z0 = a[i][0];
+                stor<SO>::st(orow, cpx<T>{z0.x + z0.y, T(0)});
+                stor<SO>::st(orow + 2 * L, cpx<T>{z0.x - z0.y, T(0)});
+#pragma unroll
+                for (int q = 1; q < RL / 2; ++q) r2c_pair<T, SO>(orow, L, q * NSL, a[i][q], a[i][RL - q], tw);
+                stor<SO>::st(orow + 2 * (L / 2), conjc(a[i][RL / 2]));
+            }
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < RL; ++q) r2c_pair<T, SO>(orow, L, j + q * NSL, a[i][q], c[i][RL - 1 - q], tw);
+            }
+        }
+        if (self)
+        {
+#pragma unroll
+            for (int q = 0; q < RL / 2; ++q) r2c_pair<T, SO>(orow, L, HALF + q * NSL, s[q], s[RL - 1 - q], tw);
+        }
+    }
+    else
+    {
+        // 16-bit storage: outputs X[k] (xa) and X[L-k] (xb), k = j + q NSL,
+        // kept until the row maximum is known
+        cpx<T> xa[B::NMAIN][RL], xb[B::NMAIN][RL], ya[RL / 2], yb[RL / 2];
+        float m = 0.0f;
+#pragma unroll
+        for (int i = 0; i < B::NMAIN; ++i)
+        {
+            const int j = t + i * NT;
+            const int jp = NSL - j;
+#pragma unroll
+            for (int r = 1; r < RL; ++r)
+            {
+                a[i][r] = cmul(a[i][r], tw[2 * r * j]);
+                c[i][r] = cmul(c[i][r], tw[(2 * r * jp) & (2 * L - 1)]);
+            }
+            dftR<T, RL>(a[i]);
+            dftR<T, RL>(c[i]);
+            if (j == 0)
+            {
+                // slot 0: X[0], X[L]; slots 1..RL/2-1: pairs; slot RL/2: X[L/2]; the rest unused
+                const cpx<T> z0 = a[i][0];
+                xa[i][0] = cpx<T>{z0.x + z0.y, T(0)};
+                xb[i][0] = cpx<T>{z0.x - z0.y, T(0)};
+#pragma unroll
+                for (int q = 1; q < RL / 2; ++q) r2c_val(a[i][q], a[i][RL - q], tw[q * NSL], xa[i][q], xb[i][q]);
+                xa[i][RL / 2] = conjc(a[i][RL / 2]);
+                xb[i][RL / 2] = xa[i][RL / 2];
+#pragma unroll
+                for (int q = RL / 2 + 1; q < RL; ++q) xa[i][q] = xb[i][q] = cpx<T>{T(0), T(0)};
+            }
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < RL; ++q) r2c_val(a[i][q], c[i][RL - 1 - q], tw[j + q * NSL], xa[i][q], xb[i][q]);
+            }
+#pragma unroll
+            for (int q = 0; q < RL; ++q) m = fmaxf(m, cmax(xa[i][q], xb[i][q]));
+        }
+        if (self)
+        {
+#pragma unroll
+            for (int q = 0; q < RL / 2; ++q)
+            {
+                r2c_val(s[q], s[RL - 1 - q], tw[HALF + q * NSL], ya[q], yb[q]);
+                m = fmaxf(m, cmax(ya[q], yb[q]));
+            }
+        }
+        atomicMax(smax, __float_as_uint(m));
+        __syncthreads();
+        const int ex = pick_exp(__uint_as_float(*smax));
+        const T sc = (T)ldexpf(1.0f, ex);
+        if (t == 0 && oexp) *oexp = ex;
+#pragma unroll
+        for (int i = 0; i < B::NMAIN; ++i)
+        {
+            const int j = t + i * NT;
+#pragma unroll
+            for (int q = 0; q < RL; ++q)
+            {
+                const int k = j + q * NSL;
+                if (j != 0 || q <= RL / 2)
+                {
+                    stor<SO>::st(orow + 2 * k, scal(xa[i][q], sc));
+                    if (j != 0 || q != RL / 2) stor<SO>::st(orow + 2 * (L - k), scal(xb[i][q], sc));
+                }
+            }
+        }
+        if (self)
+        {
+#pragma unroll
+            for (int q = 0; q < RL / 2; ++q)
+            {
+                const int k = HALF + q * NSL;
+                stor<SO>::st(orow + 2 * k, scal(ya[q], sc));
+                stor<SO>::st(orow + 2 * (L - k), scal(yb[q], sc));
+            }
+        }
     }
 }
 
-// Forward r2c of long rows (FP64 / FP32 storage = compute type): workgroup
-// g transforms row g of the nrows x batch rows.
-template <typename T, int LG>
+// Forward r2c of long rows: workgroup g transforms row g of the nrows x
+// batch rows.  SI / SO: input / output storage (T, or 16-bit with T = float).
+template <typename T, typename SI, typename SO, int LG>
 __global__ void __launch_bounds__((1 << LG) / 16)
-row_r2c_big_kernel(const T *__restrict__ in, T *__restrict__ out, long long ld_in, long long ld_out,
-                   long long dist_in, long long dist_out, int nrows, const cpx<T> *__restrict__ tw,
-                   const cpx<double> *__restrict__ tw64)
+row_r2c_big_kernel(RowArgs a)
 {
     constexpr int L = 1 << LG;
     using B = bigrow<T, L>;
     constexpr int NT = B::NT;
     __shared__ T lds[B::SPLIT ? B::LP : 2 * B::LP];
+    __shared__ unsigned smax;
     const int t = threadIdx.x;
+    if (is16<SO>() && t == 0) smax = 0u;   // ordered before its use by the stage barriers
     const int g = xcd_order((int)blockIdx.x, (int)gridDim.x);   // consecutive rows share an XCD
-    const int b = g / nrows, row = g - b * nrows;
-    const T *irow = in + (long long)b * dist_in + (long long)row * ld_in;
+    const int b = g / a.nrows, row = g - b * a.nrows;
+    const SI *irow = reinterpret_cast<const SI *>(a.in) + (long long)b * a.dist_in + (long long)row * a.ld_in;
     cpx<T> v[16];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = stor<T>::ld(irow + 2 * (t + r * NT));
-    big_row_body<T, LG>(v, lds, tw, tw64, out + (long long)b * dist_out + (long long)row * ld_out, t);
+    for (int r = 0; r < 16; ++r) v[r] = stor<SI>::ld(irow + 2 * (t + r * NT));
+    if constexpr (is16<SI>())
+    {
+        if (a.in_exp)
+        {
+            const T sc = (T)ldexpf(1.0f, -a.in_exp[(long long)b * a.exp_dist + row]);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = scal(v[r], sc);
+        }
+    }
+    SO *orow = reinterpret_cast<SO *>(a.out) + (long long)b * a.dist_out + (long long)row * a.ld_out;
+    int *oexp = (is16<SO>() && a.out_exp) ? a.out_exp + (long long)b * a.exp_dist + row : nullptr;
+    big_row_body<T, SO, LG>(v, lds, reinterpret_cast<const cpx<T> *>(a.tw),
+                            reinterpret_cast<const cpx<double> *>(a.tw64), orow, t, &smax, oexp);
 }
 
 }   // namespace hpxfft_dev

@@ -11,6 +11,14 @@
 //   MODE 2 (phase B): scratch ((strip*Q + k2)*P + e)*W + w -> DFT over m1 ->
 //          spectrum row Q*k1 + k2
 // CONJ computes the inverse transform as conj(DFT(conj(.))).
+//
+// SS is the spectrum's storage type; the scratch holds the compute type T.
+// 16-bit SS (T = float): phase A unscales row m by 2^-row_exp[m] on load and
+// folds each column's largest scratch component into colmax (atomicMax on
+// the float bit pattern, valid for non-negative values); phase B stores
+// X * 2^e per column with e = pick_exp(1.5 P colmax), since
+// |X[Q k1 + k2]| <= sum_m1 |Z[m1][k2]| <= P sqrt(2) max component, and
+// writes e to out_exp.
 #pragma once
 
 #include "fft_kernels.hpp"
@@ -60,23 +68,32 @@ template <typename T, int L, int W> struct tileinfo
     static constexpr int NT = W * L / 16;
     static constexpr int JPC = L / 16;       // threads per column
     static constexpr int NB1 = 16 / R1;      // second-stage butterflies per thread
-    // FP64 tiles exchange real and imaginary halves (half the LDS, twice the workgroups per CU)
-    static constexpr bool SPLIT = sizeof(T) == 8;
+    // FP64 tiles, and tiles whose complex exchange would exceed 64 KiB, exchange
+    // real and imaginary halves (half the LDS, twice the workgroups per CU)
+    static constexpr bool SPLIT = sizeof(T) == 8 || (long)L * W * 2 * (long)sizeof(T) > 64 * 1024;
 };
 
-template <typename T, typename SC, int LG, int MODE, bool CONJ>
-__global__ void __launch_bounds__((cfg<T, SC>::TW << LG) / 16)
+template <typename T, typename SS, int LG, int MODE, bool CONJ>
+__global__ void __launch_bounds__((cfg<T, SS>::TW << LG) / 16)
 tile_kernel(ColArgs a)
 {
     constexpr int L = 1 << LG;
-    constexpr int W = cfg<T, SC>::TW;
+    constexpr int W = cfg<T, SS>::TW;
     using TI = tileinfo<T, L, W>;
     constexpr int R1 = TI::R1, JPC = TI::JPC, NB1 = TI::NB1;
     static_assert(R1 >= 2 && R1 <= 16, "L = 32..256");
     constexpr bool SPLIT = TI::SPLIT;
+    constexpr bool S16 = is16<SS>();
+    using SIN = typename std::conditional<MODE == 2, T, SS>::type;    // scratch is the compute type
+    using SOUT = typename std::conditional<MODE == 1, T, SS>::type;
     __shared__ T lds[SPLIT ? L * W : 2 * L * W];
+    __shared__ unsigned smax[S16 ? W : 1];
 
     const int t = threadIdx.x;
+    if constexpr (S16 && MODE == 1)
+    {
+        if (t < W) smax[t] = 0u;   // ordered before the atomics by the exchange barriers
+    }
     const int w = t % W, jj = t / W;   // column in the tile, butterfly index in the column
     const blk3 bk = xcd_block();       // neighbouring strips of one sub share an XCD
     const int sub = bk.y;              // m1 (MODE 1) or k2 (MODE 2)
@@ -96,7 +113,14 @@ tile_kernel(ColArgs a)
         if constexpr (MODE == 1) idx = (long long)(sub + a.P * e) * a.pitch + col0 + w;
         else idx = sbase + ((long long)sub * a.P + e) * W + w;
         cpx<T> z = {T(0), T(0)};
-        if (MODE == 2 || valid) z = stor<T>::ld(reinterpret_cast<const T *>(a.in) + 2 * idx);
+        if (MODE == 2 || valid)
+        {
+            z = stor<SIN>::ld(reinterpret_cast<const SIN *>(a.in) + 2 * idx);
+            if constexpr (S16 && MODE == 1)
+            {
+                if (a.row_exp) z = scal(z, (T)ldexpf(1.0f, -a.row_exp[sub + a.P * e]));
+            }
+        }
         if constexpr (CONJ && MODE == 1) z = conjc(z);
         v[r] = z;
     }
@@ -185,6 +209,26 @@ tile_kernel(ColArgs a)
             }
         }
     }
+    if constexpr (S16 && MODE == 1)
+    {
+        // this thread's column w: largest scratch component -> LDS -> colmax
+        float m = 0.0f;
+#pragma unroll
+        for (int x = 0; x < 16; ++x) m = fmaxf(m, fmaxf(fabsf((float)v[x].x), fabsf((float)v[x].y)));
+        atomicMax(&smax[w], __float_as_uint(m));
+        __syncthreads();
+        if (t < W && t < a.ncols - col0 && a.colmax) atomicMax(&a.colmax[col0 + t], smax[t]);
+    }
+    T osc = T(1);
+    if constexpr (S16 && MODE == 2)
+    {
+        if (valid)
+        {
+            const int ex = pick_exp(__uint_as_float(a.colmax[col0 + w]) * 1.5f * (float)a.P);
+            osc = (T)ldexpf(1.0f, ex);
+            if (jj == 0 && a.out_exp) a.out_exp[col0 + w] = ex;   // the thread holding e = 0
+        }
+    }
 #pragma unroll
     for (int i = 0; i < NB1; ++i)
     {
@@ -196,13 +240,15 @@ tile_kernel(ColArgs a)
             cpx<T> z = v[i * R1 + q];
             if constexpr (MODE == 1)
             {
-                stor<T>::st(reinterpret_cast<T *>(a.out) + 2 * (sbase + ((long long)e * a.P + sub) * W + w), z);
+                stor<SOUT>::st(reinterpret_cast<SOUT *>(a.out) + 2 * (sbase + ((long long)e * a.P + sub) * W + w), z);
             }
             else
             {
                 if constexpr (CONJ) z = conjc(z);
+                if constexpr (S16) z = scal(z, osc);
                 if (valid)
-                    stor<T>::st(reinterpret_cast<T *>(a.out) + 2 * ((long long)(a.Q * e + sub) * a.pitch + col0 + w), z);
+                    stor<SOUT>::st(reinterpret_cast<SOUT *>(a.out) +
+                                       2 * ((long long)(a.Q * e + sub) * a.pitch + col0 + w), z);
             }
         }
     }

@@ -123,7 +123,7 @@ using namespace hpxfft_dev;
 
 template <typename T, typename S, int LG> constexpr bool use_big_row()
 {
-    return std::is_same<T, S>::value && (LG == 11 || LG == 13 || LG == 14);
+    return LG == 11 || LG == 13 || LG == 14;
 }
 template <typename T, typename S, int LG>
 hipError_t launch_row_r2c(const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
@@ -132,10 +132,7 @@ hipError_t launch_row_r2c(const RowArgs &a, unsigned gx, unsigned gy, hipStream_
     {
         (void)gx;
         const unsigned total = (unsigned)a.nrows * gy;
-        hipLaunchKernelGGL((row_r2c_big_kernel<T, LG>), dim3(total), dim3((1 << LG) / 16), 0, s,
-                           reinterpret_cast<const T *>(a.in), reinterpret_cast<T *>(a.out), a.ld_in, a.ld_out,
-                           a.dist_in, a.dist_out, a.nrows, reinterpret_cast<const cpx<T> *>(a.tw),
-                           reinterpret_cast<const cpx<double> *>(a.tw64));
+        hipLaunchKernelGGL((row_r2c_big_kernel<T, S, S, LG>), dim3(total), dim3((1 << LG) / 16), 0, s, a);
     }
     else
     {
@@ -153,9 +150,9 @@ hipError_t launch_row_c2r(const RowArgs &a, unsigned gx, unsigned gy, hipStream_
 template <typename T, typename S, int LG, int MODE, bool CONJ>
 hipError_t launch_col(const ColArgs &a, unsigned gx, unsigned gy, unsigned gz, hipStream_t s)
 {
-    if constexpr (MODE != 0 && std::is_same<T, S>::value)
+    if constexpr (MODE != 0)
     {
-        // four-step tiles, FP64 / FP32 storage: specialised two-stage kernel
+        // four-step tiles: specialised two-stage kernel (scratch in the compute type)
         (void)gz;
         hipLaunchKernelGGL((tile_kernel<T, S, LG, MODE, CONJ>), dim3(gx, gy), dim3((cfg<T, S>::TW << LG) / 16), 0, s,
                            a);
@@ -226,7 +223,7 @@ hipError_t row_c2r_maybe(int lg, const RowArgs &a, unsigned gx, unsigned gy, hip
 template <typename T, typename S> geom row_geom_t(int lg)
 {
     using C = cfg<T, S>;
-    if (std::is_same<T, S>::value && (lg == 11 || lg == 13 || lg == 14))
+    if (lg == 11 || lg == 13 || lg == 14)
         return geom{(1 << lg) / 16, 1, 16, (long)((1 << lg) + (1 << lg) / 16) * 2 * (long)sizeof(T) > 40 * 1024};
     return geom{C::row_NT(lg), C::row_NSEQ(lg), C::row_E(lg), C::row_SPLIT(lg)};
 }

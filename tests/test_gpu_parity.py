@@ -25,6 +25,7 @@ SIZES = [
     (256, 16384),        # the reference README sizes as exponents: 2^8 x 2^14
     (16384, 64), (32768, 4), (16, 32768), (65536, 2),
     (64, 4096), (1, 4096), (2, 16384), (4, 32768),
+    (32768, 256),        # 256-point four-step tiles over several strips
 ]
 
 

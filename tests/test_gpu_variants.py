@@ -39,7 +39,8 @@ def to16(x_pad, ny, dt, scale_rows):
 @pytest.mark.parametrize("dt_name,bound", [("f16", 5e-3), ("bf16", 4e-2)])
 @pytest.mark.parametrize("nx,ny,amp", [(8, 16, 1.0), (64, 256, 1.0), (256, 1024, 1e6), (1024, 512, 1.0),
                                        (4096, 64, 1e6), (512, 8192, 1.0), (64, 32768, 1.0),
-                                       (32768, 64, 1e6)])
+                                       (32768, 64, 1e6), (256, 4096, 1e6), (128, 16384, 1.0),
+                                       (32768, 256, 1.0)])
 def test_16bit_storage(gpu, dt_name, bound, nx, ny, amp):
     import torch
     import hpxfft_amd
