@@ -596,12 +596,19 @@ int hpxfft_exec_r2c_2d_host(hpxfft_plan plan, double *inout)
     return HPXFFT_OK;
 }
 
-int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_real)
+}   // extern "C"
+
+namespace {
+
+// spectrum -> real through an inverse column c2c into the plan's work buffer,
+// then a row c2r: the r2c adjoint (halve) or the unnormalised inverse c2r
+int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool halve)
 {
+    const char *what = halve ? "adjoint" : "c2r";
     if (!plan || plan->batched) return fail(HPXFFT_INVALID_ARG, "invalid plan");
     if (!d_spec || !d_real) return fail(HPXFFT_INVALID_ARG, "null device pointer");
     if (plan->prec != P_F64 && plan->prec != P_F32)
-        return fail(HPXFFT_INVALID_ARG, "adjoint is implemented for F64 and F32 plans");
+        return fail(HPXFFT_INVALID_ARG, std::string(what) + " is implemented for F64 and F32 plans");
     device_guard g(plan->device);
     const size_t sb = storage_bytes(plan->prec);
     const size_t bytes = plan->batch * plan->dist_out * 2 * sb;
@@ -626,11 +633,25 @@ int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_rea
     r.tw64 = tw64_ptr(plan, plan->ny);
     const geom rg = row_geom(plan->prec, plan->lgL);
     const unsigned gx = (unsigned)((plan->nx + rg.nseq - 1) / rg.nseq);
-    HIPCHK(row_c2r(plan->prec, plan->lgL, r, gx, (unsigned)plan->batch, plan->stream));
+    HIPCHK(row_c2r(plan->prec, plan->lgL, halve, r, gx, (unsigned)plan->batch, plan->stream));
     HIPCHK(hipEventRecord(plan->ev[2], plan->stream));
     plan->ran = true;
     plan->col_split = false;
     return HPXFFT_OK;
+}
+
+}   // namespace
+
+extern "C" {
+
+int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_real)
+{
+    return exec_backward(plan, d_spec, d_real, true);
+}
+
+int hpxfft_exec_c2r_2d(hpxfft_plan plan, const void *d_spec, void *d_real)
+{
+    return exec_backward(plan, d_spec, d_real, false);
 }
 
 int hpxfft_get_measurement(hpxfft_plan plan, const char *key, double *value)

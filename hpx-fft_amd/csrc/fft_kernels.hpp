@@ -278,8 +278,8 @@ __device__ __forceinline__ void col_body(const ColArgs &a, long long b, int sub,
 
     const SI *in = reinterpret_cast<const SI *>(a.in);
     SO *out = reinterpret_cast<SO *>(a.out);
-    if constexpr (MODE != 2) in += b * a.dist;
-    if constexpr (MODE != 1) out += b * a.dist;
+    if constexpr (MODE != 2) in += 2 * b * a.dist;   // dist counts complex elements
+    if constexpr (MODE != 1) out += 2 * b * a.dist;
 
     cpx<T> v[E];
 #pragma unroll
@@ -389,12 +389,15 @@ col_kernel(ColArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// Row c2r for the adjoint.  Input: half-spectrum rows G[0..L] (complex, SI),
-// output: real rows (SO).  out[2n] + i out[2n+1] = IDFT_L(Z),
-//   H = G/2 on 0<k<L, H[0] = Re G[0], H[L] = Re G[L],
+// Row c2r.  Input: half-spectrum rows G[0..L] (complex, SI), output: real
+// rows (SO).  out[2n] + i out[2n+1] = IDFT_L(Z),
+//   H = G on 0<k<L (halved when HALVE), H[0] = Re G[0], H[L] = Re G[L],
 //   Z[k] = (H[k] + conj H[L-k]) + i (H[k] - conj H[L-k]) conj(w_{2L}^k).
-// The inverse DFT is computed as conj(DFT(conj Z)).
-template <typename T, typename SI, typename SO, int LG>
+// Without halving this is the unnormalised c2r of length 2L (FFTW_BACKWARD,
+// Hermitian extension of G); with halving it is the row half of the r2c
+// adjoint under Re<A,B> on the half spectrum.  The inverse DFT is computed
+// as conj(DFT(conj Z)).
+template <typename T, typename SI, typename SO, int LG, bool HALVE>
 __global__ void __launch_bounds__((cfg<T, SI>::row_NT(LG)))
 row_c2r_kernel(RowArgs a)
 {
@@ -440,7 +443,8 @@ row_c2r_kernel(RowArgs a)
         }
         else
         {
-            const cpx<T> h0 = scal(g0, T(0.5)), h1 = scal(g1, T(0.5));
+            const T hs = HALVE ? T(0.5) : T(1);
+            const cpx<T> h0 = scal(g0, hs), h1 = scal(g1, hs);
             const cpx<T> w = tw[k];
             const cpx<T> wc = conjc(w);
             // Z[k]

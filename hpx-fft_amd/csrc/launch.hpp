@@ -28,7 +28,7 @@ struct geom
 // per precision (defined in inst_<prec>.hip)
 #define HPXFFT_DECL_LAUNCH(sfx)                                                                                    \
     hipError_t row_r2c_##sfx(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s);                   \
-    hipError_t row_c2r_##sfx(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s);                   \
+    hipError_t row_c2r_##sfx(int lg, bool halve, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s);       \
     hipError_t col_##sfx(int lg, int mode, bool conj, const ColArgs &a, unsigned gx, unsigned gy, unsigned gz,    \
                          hipStream_t s);                                                                           \
     geom row_geom_##sfx(int lg);                                                                                   \
@@ -51,12 +51,13 @@ inline hipError_t row_r2c(int p, int lg, const RowArgs &a, unsigned gx, unsigned
     default: return row_r2c_bf16(lg, a, gx, gy, s);
     }
 }
-inline hipError_t row_c2r(int p, int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+// halve: adjoint (interior bins halved); otherwise the unnormalised inverse c2r
+inline hipError_t row_c2r(int p, int lg, bool halve, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
 {
     switch (p)
     {
-    case P_F64: return row_c2r_f64(lg, a, gx, gy, s);
-    case P_F32: return row_c2r_f32(lg, a, gx, gy, s);
+    case P_F64: return row_c2r_f64(lg, halve, a, gx, gy, s);
+    case P_F32: return row_c2r_f32(lg, halve, a, gx, gy, s);
     default: return hipErrorInvalidValue;
     }
 }
@@ -142,9 +143,12 @@ hipError_t launch_row_r2c(const RowArgs &a, unsigned gx, unsigned gy, hipStream_
 }
 
 template <typename T, typename S, int LG>
-hipError_t launch_row_c2r(const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+hipError_t launch_row_c2r(bool halve, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
 {
-    hipLaunchKernelGGL((row_c2r_kernel<T, S, S, LG>), dim3(gx, gy), dim3(cfg<T, S>::row_NT(LG)), 0, s, a);
+    if (halve)
+        hipLaunchKernelGGL((row_c2r_kernel<T, S, S, LG, true>), dim3(gx, gy), dim3(cfg<T, S>::row_NT(LG)), 0, s, a);
+    else
+        hipLaunchKernelGGL((row_c2r_kernel<T, S, S, LG, false>), dim3(gx, gy), dim3(cfg<T, S>::row_NT(LG)), 0, s, a);
     return hipGetLastError();
 }
 template <typename T, typename S, int LG, int MODE, bool CONJ>
@@ -175,10 +179,12 @@ hipError_t row_r2c_sw(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStr
     else return lg == LG ? launch_row_r2c<T, S, LG>(a, gx, gy, s) : row_r2c_sw<T, S, LG + 1>(lg, a, gx, gy, s);
 }
 template <typename T, typename S, int LG = 0>
-hipError_t row_c2r_sw(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+hipError_t row_c2r_sw(int lg, bool halve, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
 {
     if constexpr (LG > ROW_MAX_LG) return hipErrorInvalidValue;
-    else return lg == LG ? launch_row_c2r<T, S, LG>(a, gx, gy, s) : row_c2r_sw<T, S, LG + 1>(lg, a, gx, gy, s);
+    else
+        return lg == LG ? launch_row_c2r<T, S, LG>(halve, a, gx, gy, s)
+                        : row_c2r_sw<T, S, LG + 1>(lg, halve, a, gx, gy, s);
 }
 template <typename T, typename S, bool ADJ, int LG = 0>
 hipError_t col_sw(int lg, int mode, bool conj, const ColArgs &a, unsigned gx, unsigned gy, unsigned gz, hipStream_t s)
@@ -214,9 +220,9 @@ hipError_t col_sw(int lg, int mode, bool conj, const ColArgs &a, unsigned gx, un
 }
 
 template <typename T, typename S, bool ADJ>
-hipError_t row_c2r_maybe(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
+hipError_t row_c2r_maybe(int lg, bool halve, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
 {
-    if constexpr (ADJ) return row_c2r_sw<T, S>(lg, a, gx, gy, s);
+    if constexpr (ADJ) return row_c2r_sw<T, S>(lg, halve, a, gx, gy, s);
     else return hipErrorInvalidValue;
 }
 
@@ -246,9 +252,9 @@ template <typename T, typename S> geom tile_geom_t(int lg)
     {                                                                                                              \
         return detail::row_r2c_sw<T, S>(lg, a, gx, gy, s);                                                         \
     }                                                                                                              \
-    hipError_t row_c2r_##sfx(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)                    \
+    hipError_t row_c2r_##sfx(int lg, bool halve, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)        \
     {                                                                                                              \
-        return detail::row_c2r_maybe<T, S, ADJ>(lg, a, gx, gy, s);                                                 \
+        return detail::row_c2r_maybe<T, S, ADJ>(lg, halve, a, gx, gy, s);                                          \
     }                                                                                                              \
     hipError_t col_##sfx(int lg, int mode, bool conj, const ColArgs &a, unsigned gx, unsigned gy, unsigned gz,     \
                          hipStream_t s)                                                                            \

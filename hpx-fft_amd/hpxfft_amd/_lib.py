@@ -26,7 +26,7 @@ ESTIMATE, MEASURE, PATIENT, EXHAUSTIVE = range(4)
 # every entry point declared in include/hpxfft_hip.h
 EXPORTS = (
     "hpxfft_plan_flag_from_string", "hpxfft_plan_r2c_2d", "hpxfft_plan_r2c_2d_many", "hpxfft_exec_r2c_2d",
-    "hpxfft_exec_r2c_2d_scaled", "hpxfft_exec_r2c_2d_host", "hpxfft_exec_r2c_2d_adjoint", "hpxfft_get_measurement",
+    "hpxfft_exec_r2c_2d_scaled", "hpxfft_exec_r2c_2d_host", "hpxfft_exec_r2c_2d_adjoint", "hpxfft_exec_c2r_2d", "hpxfft_get_measurement",
     "hpxfft_write_plan", "hpxfft_plan_describe", "hpxfft_plan_destroy", "hpxfft_plan_r2c_2d_batched",
     "hpxfft_exec_r2c_2d_batched", "hpxfft_last_error", "hpxfft_version",
 )
@@ -58,6 +58,7 @@ def lib() -> ctypes.CDLL:
     L.hpxfft_exec_r2c_2d_scaled.argtypes = [vp, vp, vp, vp, vp]
     L.hpxfft_exec_r2c_2d_host.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.hpxfft_exec_r2c_2d_adjoint.argtypes = [vp, vp, vp]
+    L.hpxfft_exec_c2r_2d.argtypes = [vp, vp, vp]
     L.hpxfft_get_measurement.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
     L.hpxfft_write_plan.argtypes = [vp, ctypes.c_char_p]
     L.hpxfft_plan_describe.argtypes = [vp, ctypes.c_char_p, sz, ctypes.POINTER(sz)]

@@ -70,6 +70,10 @@ class Plan:
     def exec_adjoint(self, d_spec: int, d_real: int) -> None:
         check(lib().hpxfft_exec_r2c_2d_adjoint(self._h, ctypes.c_void_p(d_spec), ctypes.c_void_p(d_real)))
 
+    def exec_c2r(self, d_spec: int, d_real: int) -> None:
+        """Unnormalised inverse (FFTW_BACKWARD): exec_c2r after exec gives Nx*Ny*x."""
+        check(lib().hpxfft_exec_c2r_2d(self._h, ctypes.c_void_p(d_spec), ctypes.c_void_p(d_real)))
+
     def exec_host(self, values: np.ndarray) -> None:
         assert values.dtype == np.float64 and values.flags.c_contiguous
         check(lib().hpxfft_exec_r2c_2d_host(self._h, values.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))

@@ -4,6 +4,8 @@ current HIP stream; all arithmetic runs in libhpxfft_hip.so).
 * :func:`rfft2`            -- forward 2-D r2c of (..., Nx, Ny) real tensors.
 * :func:`rfft2_scaled`     -- 16-bit storage variant with power-of-two scales.
 * :func:`rfft2_adjoint`    -- the adjoint R* of the r2c map.
+* :func:`c2r_2d`           -- the unnormalised inverse (FFTW_BACKWARD):
+                              c2r_2d(rfft2(x), Ny) = Nx*Ny*x.
 * :class:`RFFT2`           -- torch.autograd.Function: forward = rfft2,
                               backward = rfft2_adjoint (hand-written kernels).
 """
@@ -61,6 +63,17 @@ def rfft2(x: torch.Tensor) -> torch.Tensor:
 
 def rfft2_adjoint(g: torch.Tensor, ny: int) -> torch.Tensor:
     """(R* g)[..., m, n] = Re sum_{kx, ky<=Ny/2} g[..., kx, ky] exp(+2 pi i (kx m/Nx + ky n/Ny))."""
+    return _backward(g, ny, adjoint=True)
+
+
+def c2r_2d(X: torch.Tensor, ny: int) -> torch.Tensor:
+    """Unnormalised 2-D c2r (FFTW_BACKWARD) of a half spectrum (..., Nx, Ny/2+1):
+    x = Nx*Ny * numpy.fft.irfft2(X, s=(Nx, Ny)) (imaginary parts of the ky = 0
+    and ky = Ny/2 bins ignored after the column transform)."""
+    return _backward(X, ny, adjoint=False)
+
+
+def _backward(g: torch.Tensor, ny: int, adjoint: bool) -> torch.Tensor:
     if not g.is_cuda:
         raise ValueError("hpxfft_amd operates on GPU tensors (no CPU path)")
     rdt = {torch.complex128: torch.float64, torch.complex64: torch.float32}.get(g.dtype)
@@ -75,7 +88,10 @@ def rfft2_adjoint(g: torch.Tensor, ny: int) -> torch.Tensor:
     if batch == 0:
         return out
     p = _get_plan(nx, ny, batch, _DT[rdt][0], ny, ny // 2 + 1, g.device)
-    p.exec_adjoint(g.data_ptr(), out.data_ptr())
+    if adjoint:
+        p.exec_adjoint(g.data_ptr(), out.data_ptr())
+    else:
+        p.exec_c2r(g.data_ptr(), out.data_ptr())
     return out
 
 

@@ -110,6 +110,16 @@ int hpxfft_exec_r2c_2d_host(hpxfft_plan plan, double *inout);
  * and is not modified; d_real uses the plan's input layout.  F64/F32 plans. */
 int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_real);
 
+/* Inverse 2-D c2r, unnormalised, in FFTW's FFTW_BACKWARD convention: the
+ * reference declares the backward direction (core/include/hpxfft/util/
+ * adapter_fftw.hpp:11) but never plans it.  x[m][n] = sum_kx sum_ky
+ * X[kx][ky] exp(+2 pi i (kx m/Nx + ky n/Ny)) over the Hermitian extension of
+ * the half spectrum, the imaginary parts of the ky = 0 and ky = Ny/2 bins
+ * ignored after the column transform (as numpy.fft.irfft2 times Nx*Ny), so
+ * c2r(r2c(x)) = Nx*Ny*x.  d_spec uses the plan's output layout and is not
+ * modified; d_real uses the plan's input layout.  F64/F32 plans. */
+int hpxfft_exec_c2r_2d(hpxfft_plan plan, const void *d_spec, void *d_real);
+
 /* Replaces hpxfft::shared::loop::get_measurement (loop.cpp:192): seconds of
  * the last exec for "total", "first_fftw", "first_trans", "second_fftw",
  * "second_trans" (the transposes are fused into the column pass and read

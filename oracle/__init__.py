@@ -196,6 +196,17 @@ def adjoint_r2c_2d(Y: np.ndarray, ny: int, nthreads: int = 0) -> np.ndarray:
     return out
 
 
+def c2r_2d(X: np.ndarray, ny: int, nthreads: int = 0) -> np.ndarray:
+    """Unnormalised inverse 2-D c2r (FFTW_BACKWARD, the direction the reference
+    declares at core/include/hpxfft/util/adapter_fftw.hpp:11): the Hermitian
+    extension of the half spectrum, i.e. R*(c X) with c(ky) = 2 for
+    0 < ky < Ny/2 and 1 at ky = 0, Ny/2 (R* halves exactly those bins)."""
+    X = np.array(X, dtype=np.complex128)
+    if ny >= 4:
+        X[:, 1:ny // 2] *= 2.0
+    return adjoint_r2c_2d(X, ny, nthreads)
+
+
 def rel_l2(a: np.ndarray, ref: np.ndarray) -> float:
     a = np.asarray(a, dtype=np.complex128)
     ref = np.asarray(ref, dtype=np.complex128)

@@ -63,6 +63,40 @@ def test_parity_f64(gpu, nx, ny):
     assert err <= TOL64, err
 
 
+@pytest.mark.parametrize("nx,ny,batch", [(128, 256, 3), (1024, 64, 2), (4096, 128, 3), (16, 4096, 5)])
+@pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
+def test_parity_batched_strided(gpu, nx, ny, batch, prec, tol):
+    """Several grids in one plan, out of place, with gaps between grids and
+    between rows (dist_in > Nx*ld_in, ld_out > Ny/2+1): every grid against
+    the oracle, and the gaps left untouched."""
+    import torch
+    import hpxfft_amd
+    print(f"batched {prec} {batch} x {nx}x{ny}", flush=True)
+    ncy = ny // 2 + 1
+    ld_in, ld_out = 2 * ncy + 6, ncy + 3
+    dist_in, dist_out = nx * ld_in + 10, nx * ld_out + 7
+    xs = [oracle.fill(nx, ny, seed=3000 + b, dist="upm1") for b in range(batch)]
+    dt = torch.float64 if prec == "f64" else torch.float32
+    cdt = torch.complex128 if prec == "f64" else torch.complex64
+    host_in = np.zeros(batch * dist_in)
+    for b, x in enumerate(xs):
+        host_in[b * dist_in: b * dist_in + nx * ld_in].reshape(nx, ld_in)[:, :ny] = x[:, :ny]
+    d_in = torch.from_numpy(host_in).to(device="cuda", dtype=dt)
+    d_out = torch.full((batch * dist_out,), complex(7.0, -7.0), dtype=cdt, device="cuda")
+    p = hpxfft_amd.Plan(nx, ny, batch=batch, ld_in=ld_in, dist_in=dist_in, ld_out=ld_out, dist_out=dist_out,
+                        precision=prec)
+    p.exec(d_in.data_ptr(), d_out.data_ptr())
+    torch.cuda.synchronize()
+    out = d_out.cpu().numpy().astype(np.complex128)
+    p.close()
+    for b, x in enumerate(xs):
+        grid = out[b * dist_out: b * dist_out + nx * ld_out].reshape(nx, ld_out)
+        err = oracle.rel_l2(grid[:, :ncy], oracle.r2c_2d(x, ny))
+        assert err <= tol, (b, err)
+        assert np.all(grid[:, ncy:] == complex(7.0, -7.0)), b
+        assert np.all(out[b * dist_out + nx * ld_out: (b + 1) * dist_out] == complex(7.0, -7.0)), b
+
+
 @pytest.mark.parametrize("nx,ny", SIZES)
 def test_parity_f32(gpu, nx, ny):
     print(f"f32 {nx}x{ny}", flush=True)
