@@ -185,7 +185,8 @@ struct hpxfft_plan_s
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     int *mid_exp = nullptr;       // 16-bit: per-row exponents after the row pass
-    unsigned *colmax = nullptr;   // 16-bit four-step: per-column max
+    unsigned *colpart = nullptr;  // 16-bit four-step: per-workgroup column maxima, P x ncols
+    unsigned *colmax = nullptr;   // 16-bit four-step: per-column maxima, ncols
     int *col_exp_own = nullptr;   // 16-bit: output exponents when the caller gives none
     void *adj_work = nullptr;     // adjoint: column-pass result
     size_t adj_bytes = 0;
@@ -314,14 +315,8 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
     a.P = P;
     a.Q = Q;
     a.tw_N = tw_ptr(p, p->nx);
-    if (p->colmax)
-    {
-        const long long n = (long long)(p->ncols * p->batch);
-        hipLaunchKernelGGL(hpxfft_dev::zero_u32_kernel<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, p->stream,
-                           p->colmax, n);
-        HIPCHK(hipGetLastError());
-        a.colmax = p->colmax;
-    }
+    a.colpart = p->colpart;
+    a.colmax = p->colmax;
     const unsigned gx = (unsigned)((p->ncols + W - 1) / W);
     for (size_t b = 0; b < p->batch; ++b)
     {
@@ -331,18 +326,28 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         ab.dist = 0;
         if (ab.row_exp) ab.row_exp += b * p->nx;
         if (ab.out_exp) ab.out_exp += b * p->ncols;
-        if (ab.colmax) ab.colmax += b * p->ncols;
         // phase A: spectrum -> scratch
         ab.in = in_b;
         ab.out = p->scratch;
         ab.tw_L = tw_ptr(p, (size_t)Q);
         HIPCHK(col(p->prec, p->lgQ, 1, conj, ab, gx, (unsigned)P, 1, p->stream));
+        if (p->colpart)
+        {
+            // 16-bit: fold phase A's per-workgroup maxima into one per column (every entry is rewritten)
+            hipLaunchKernelGGL(hpxfft_dev::colmax_reduce_kernel<0>, dim3((unsigned)((p->ncols + 255) / 256)), dim3(256),
+                               0, p->stream, p->colpart, p->colmax, (int)p->ncols, P);
+            HIPCHK(hipGetLastError());
+        }
         if (mid && p->batch == 1) HIPCHK(hipEventRecord(mid, p->stream));
-        // phase B: scratch -> spectrum
+        // phase B: scratch -> spectrum; its tiles are shifted to whole 128-byte
+        // lines of the output rows, which can take one more tile per row
+        const int eline = (int)(128 / (2 * sb));
         ab.in = p->scratch;
         ab.out = out_b;
+        ab.out_line_off = (int)((reinterpret_cast<uintptr_t>(out_b) / (2 * sb)) % (uintptr_t)eline);
         ab.tw_L = tw_ptr(p, (size_t)P);
-        HIPCHK(col(p->prec, p->lgP, 2, conj, ab, gx, (unsigned)Q, 1, p->stream));
+        const unsigned gxb = (unsigned)((p->ncols + eline - 2) / W + 1);
+        HIPCHK(col(p->prec, p->lgP, 2, conj, ab, gxb, (unsigned)Q, 1, p->stream));
     }
     return HPXFFT_OK;
 }
@@ -422,7 +427,12 @@ int plan_common(hpxfft_plan_s *p)
     {
         HIPCHK(hipMalloc(&p->mid_exp, sizeof(int) * p->nx * p->batch));
         HIPCHK(hipMalloc(&p->col_exp_own, sizeof(int) * p->ncols * p->batch));
-        if (p->four_step) HIPCHK(hipMalloc(&p->colmax, sizeof(unsigned) * p->ncols * p->batch));
+        if (p->four_step)
+        {
+            // reused by each grid of a batch in turn (the column pass runs grid by grid)
+            HIPCHK(hipMalloc(&p->colpart, sizeof(unsigned) * p->ncols * ((size_t)1 << p->lgP)));
+            HIPCHK(hipMalloc(&p->colmax, sizeof(unsigned) * p->ncols));
+        }
     }
     (void)sb;
     if (int rc = alloc_events(p)) return rc;
@@ -443,6 +453,7 @@ void free_plan(hpxfft_plan_s *p)
         if (e) (void)hipEventDestroy(e);
     (void)hipFree(p->scratch);
     (void)hipFree(p->mid_exp);
+    (void)hipFree(p->colpart);
     (void)hipFree(p->colmax);
     (void)hipFree(p->col_exp_own);
     (void)hipFree(p->adj_work);

@@ -127,10 +127,8 @@ __device__ __attribute__((noinline)) void bcol_task(const BGrid &g, int strip, c
     a.out = g.data;
     a.pitch = ncols;
     a.ncols = ncols;
-    a.P = 1;
-    a.Q = N;
     a.tw_L = tw32 + 2 * N;
-    col_body<float, float, float, N, C::E, W, false, 0, false>(a, 0, 0, strip, lds, nullptr);
+    col_body<float, float, float, N, C::E, W, false, false>(a, 0, strip, lds, nullptr);
 }
 
 template <int CFG>

@@ -4,12 +4,11 @@
 //                    (core/src/shared/loop.cpp:61-69): 1-D r2c of length Ny
 //                    on every row, as an Ny/2-point complex Stockham FFT of
 //                    z[n] = x[2n] + i x[2n+1] plus the r2c split, in VGPRs/LDS.
-//   col_kernel     : phases 2-4 (loop.cpp:72-101): the transposes are fused
-//                    into the column kernel's row-segment loads/stores, and
-//                    the length-Nx c2c runs either in one tile (MODE 0) or as
-//                    a four-step Nx = P*Q split into MODE 1 (Q-point DFTs
-//                    over rows m1 + P*m2, twiddle w_Nx^{m1 k2}, to scratch)
-//                    and MODE 2 (P-point DFTs, to rows Q*k1 + k2).
+//   col_kernel     : phases 2-4 (loop.cpp:72-101) for Nx <= 1024: the
+//                    transposes are fused into the column kernel's
+//                    row-segment loads/stores, and the length-Nx c2c runs in
+//                    one workgroup per strip of columns.  Longer columns use
+//                    the four-step tiles of fft_tile.hpp.
 //   row_c2r_kernel : the row half of the adjoint (no reference counterpart):
 //                    halve interior bins, keep Re of DC/Nyquist, unnormalised
 //                    c2r of length Ny.
@@ -53,10 +52,6 @@ template <typename T, typename S> struct cfg
     static constexpr int tile_E(int lg) { return 16; }
     static constexpr int tile_NT(int lg) { return (TW << lg) / 16; }
     static constexpr bool tile_SPLIT(int lg) { return (long)(TW << lg) * 2 * (long)sizeof(T) > lds_budget; }
-    template <int MODE> static constexpr int width() { return MODE == 0 ? W : TW; }
-    template <int MODE> static constexpr int nt(int lg) { return MODE == 0 ? col_NT(lg) : tile_NT(lg); }
-    template <int MODE> static constexpr int ept(int lg) { return MODE == 0 ? col_E(lg) : tile_E(lg); }
-    template <int MODE> static constexpr bool split(int lg) { return MODE == 0 ? col_SPLIT(lg) : tile_SPLIT(lg); }
 };
 
 struct RowArgs
@@ -83,13 +78,15 @@ struct ColArgs
     long long dist;    // complex elements between spectrum grids (batch)
     int ncols;         // valid columns of the spectrum (Ny/2+1)
     int P, Q;          // four-step split Nx = P*Q; scratch index ((strip*Q + k2)*P + m1)*TW + w
+    int out_line_off;  // four-step phase B: (output address / complex size) mod elements per 128-byte line
     const void *tw_L;  // w_L^m, m < L (sub-FFT length)
     const void *tw_N;  // w_Nx^m, m < Nx (MODE 1 post-twiddle)
     const int *row_exp;    // 16-bit inputs (MODE 0,1): per-row exponent of the rows
-    unsigned *colmax;      // MODE 1 writes (atomicMax of max |component| bits), MODE 2 reads
+    unsigned *colpart;     // 16-bit four-step phase A: per-workgroup column maxima [m1][ncols] (float bits)
+    unsigned *colmax;      // 16-bit four-step phase B: per-column maxima (reduced from colpart)
     int *out_exp;          // 16-bit outputs (MODE 0,2): per-column exponent
     long long rexp_dist;   // between grids for row_exp
-    long long cexp_dist;   // between grids for out_exp / colmax
+    long long cexp_dist;   // between grids for out_exp
 };
 
 template <typename S> constexpr bool is16() { return sizeof(S) == 2; }
@@ -261,25 +258,23 @@ row_r2c_kernel(RowArgs a)
 }
 
 // ---------------------------------------------------------------------------
-// column tile kernel.  blockIdx.x = strip of W columns inside the launch's
-// chunk, blockIdx.y = m1 (MODE 1) / k2 (MODE 2) / 0, blockIdx.z = grid.
-// one tile: strip of W columns (x all L elements), sub = m1 / k2 / 0, grid b.
-// Shared by the column kernel and the batched column-phase kernel.
-template <typename T, typename SI, typename SO, int L, int E, int W, bool SPLIT, int MODE, bool CONJ>
-__device__ __forceinline__ void col_body(const ColArgs &a, long long b, int sub, int strip,
+// Direct column pass: one workgroup transforms a strip of W adjacent columns
+// (one 128-byte row segment of storage) over all L = Nx rows of grid b, in
+// place or from `in` to `out` (same layout).  Shared by the column kernel
+// and the batched column-phase kernel.  16-bit storage: rows are unscaled by
+// 2^-row_exp[m] on load, and each output column is stored as X * 2^e with e
+// from its exact maximum, written to out_exp.
+template <typename T, typename SI, typename SO, int L, int E, int W, bool SPLIT, bool CONJ>
+__device__ __forceinline__ void col_body(const ColArgs &a, long long b, int strip,
                                          typename Engine<T, L, E, W, true, SPLIT>::lds_t *lds, unsigned *smax)
 {
     using EN = Engine<T, L, E, W, true, SPLIT>;
-    constexpr bool OUT16 = is16<SO>() && MODE != 1;
+    constexpr bool OUT16 = is16<SO>();
     const int t = threadIdx.x;
     const int col0 = strip * W;
     const int nvalid = min(W, a.ncols - col0);
-    const long long sbase = (long long)strip * a.P * a.Q * W;   // this strip's scratch block
-
-    const SI *in = reinterpret_cast<const SI *>(a.in);
-    SO *out = reinterpret_cast<SO *>(a.out);
-    if constexpr (MODE != 2) in += 2 * b * a.dist;   // dist counts complex elements
-    if constexpr (MODE != 1) out += 2 * b * a.dist;
+    const SI *in = reinterpret_cast<const SI *>(a.in) + 2 * b * a.dist;   // dist counts complex elements
+    SO *out = reinterpret_cast<SO *>(a.out) + 2 * b * a.dist;
 
     cpx<T> v[E];
 #pragma unroll
@@ -290,39 +285,21 @@ __device__ __forceinline__ void col_body(const ColArgs &a, long long b, int sub,
         cpx<T> z = {T(0), T(0)};
         if (w < nvalid)
         {
-            long long idx;
-            int row = 0;
-            if constexpr (MODE == 0) { row = e; idx = (long long)row * a.pitch + col0 + w; }
-            else if constexpr (MODE == 1) { row = sub + a.P * e; idx = (long long)row * a.pitch + col0 + w; }
-            else { idx = sbase + ((long long)sub * a.P + e) * W + w; }
-            z = stor<SI>::ld(in + 2 * idx);
-            if constexpr (is16<SI>() && MODE != 2)
+            z = stor<SI>::ld(in + 2 * ((long long)e * a.pitch + col0 + w));
+            if constexpr (is16<SI>())
             {
-                if (a.row_exp) z = scal(z, (T)ldexpf(1.0f, -a.row_exp[b * a.rexp_dist + row]));
+                if (a.row_exp) z = scal(z, (T)ldexpf(1.0f, -a.row_exp[b * a.rexp_dist + e]));
             }
-            if constexpr (CONJ && MODE != 2) z = conjc(z);
+            if constexpr (CONJ) z = conjc(z);
         }
         v[x] = z;
     }
 
     EN::run(v, lds, reinterpret_cast<const cpx<T> *>(a.tw_L), 1, t);
 
-    if constexpr (MODE == 1)
+    if constexpr (OUT16)
     {
-        const cpx<T> *twN = reinterpret_cast<const cpx<T> *>(a.tw_N);
-#pragma unroll
-        for (int x = 0; x < E; ++x)
-        {
-            int w, e;
-            EN::out_elem(t, x, w, e);
-            if (sub != 0) v[x] = cmul(v[x], twN[(long long)sub * e]);
-        }
-    }
-
-    // per-column maxima (16-bit outputs need a per-column exponent)
-    constexpr bool CA16 = MODE == 1 && is16<SI>();
-    if constexpr (CA16 || OUT16)
-    {
+        // per-column maxima for the output exponents
         for (int s = t; s < W; s += EN::NT) smax[s] = 0u;
         __syncthreads();
 #pragma unroll
@@ -334,10 +311,6 @@ __device__ __forceinline__ void col_body(const ColArgs &a, long long b, int sub,
             atomicMax(&smax[w], __float_as_uint(m));
         }
         __syncthreads();
-        if constexpr (CA16)
-        {
-            if (a.colmax && t < nvalid) atomicMax(&a.colmax[b * a.cexp_dist + col0 + t], smax[t]);
-        }
     }
 
 #pragma unroll
@@ -348,44 +321,32 @@ __device__ __forceinline__ void col_body(const ColArgs &a, long long b, int sub,
         if (w < nvalid)
         {
             cpx<T> z = v[x];
-            long long idx;
-            if constexpr (MODE == 0) idx = (long long)e * a.pitch + col0 + w;
-            else if constexpr (MODE == 1) idx = sbase + ((long long)e * a.P + sub) * W + w;
-            else idx = ((long long)a.Q * e + sub) * a.pitch + col0 + w;
-            if constexpr (CONJ && MODE != 1) z = conjc(z);
+            if constexpr (CONJ) z = conjc(z);
             if constexpr (OUT16)
             {
-                int ex;
-                if constexpr (MODE == 0) ex = pick_exp(__uint_as_float(smax[w]));
-                else
-                {
-                    // bound |X[Q k1 + k2]| <= sum_m1 |Z[m1][k2]| <= P * sqrt(2) * max component of Z
-                    const float cm = __uint_as_float(a.colmax[b * a.cexp_dist + col0 + w]);
-                    ex = pick_exp(cm * 1.5f * (float)a.P);
-                }
+                const int ex = pick_exp(__uint_as_float(smax[w]));
                 z = scal(z, (T)ldexpf(1.0f, ex));
                 if (e == 0 && a.out_exp) a.out_exp[b * a.cexp_dist + col0 + w] = ex;
             }
-            stor<SO>::st(out + 2 * idx, z);
+            stor<SO>::st(out + 2 * ((long long)e * a.pitch + col0 + w), z);
         }
     }
 }
 
-
-template <typename T, typename SI, typename SO, typename SC, int LG, int MODE, bool CONJ>
-__global__ void __launch_bounds__((cfg<T, SC>::template nt<MODE>(LG)))
+// blockIdx.x = strip, blockIdx.z = grid (XCD-contiguous order)
+template <typename T, typename S, int LG, bool CONJ>
+__global__ void __launch_bounds__((cfg<T, S>::col_NT(LG)))
 col_kernel(ColArgs a)
 {
     constexpr int L = 1 << LG;
-    using C = cfg<T, SC>;   // SC = spectrum storage type: fixes the tile width W
-    constexpr int W = C::template width<MODE>(), E = C::template ept<MODE>(LG);
-    constexpr bool SPLIT = C::template split<MODE>(LG);
+    using C = cfg<T, S>;
+    constexpr int W = C::W, E = C::col_E(LG);
+    constexpr bool SPLIT = C::col_SPLIT(LG);
     using EN = Engine<T, L, E, W, true, SPLIT>;
     __shared__ typename EN::lds_t lds[EN::LDS_ELEMS];
-    constexpr bool OUT16 = is16<SO>() && MODE != 1;
-    __shared__ unsigned smax[(OUT16 || (MODE == 1 && is16<SI>())) ? W : 1];
+    __shared__ unsigned smax[is16<S>() ? W : 1];
     const blk3 bk = xcd_block();
-    col_body<T, SI, SO, L, E, W, SPLIT, MODE, CONJ>(a, bk.z, bk.y, bk.x, lds, smax);
+    col_body<T, S, S, L, E, W, SPLIT, CONJ>(a, bk.z, bk.x, lds, smax);
 }
 
 // ---------------------------------------------------------------------------
@@ -525,6 +486,17 @@ template <int UNUSED = 0> __global__ void zero_u32_kernel(unsigned *p, long long
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+
+// colmax[c] = max over m1 < P of colpart[m1 * ncols + c] (non-negative float
+// bit patterns order as unsigned integers)
+template <int UNUSED = 0> __global__ void colmax_reduce_kernel(const unsigned *colpart, unsigned *colmax, int ncols, int P)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncols) return;
+    unsigned m = 0u;
+    for (int i = 0; i < P; ++i) m = max(m, colpart[(long long)i * ncols + c]);
+    colmax[c] = m;
 }
 
 }   // namespace hpxfft_dev

@@ -14,11 +14,11 @@
 //
 // SS is the spectrum's storage type; the scratch holds the compute type T.
 // 16-bit SS (T = float): phase A unscales row m by 2^-row_exp[m] on load and
-// folds each column's largest scratch component into colmax (atomicMax on
-// the float bit pattern, valid for non-negative values); phase B stores
-// X * 2^e per column with e = pick_exp(1.5 P colmax), since
-// |X[Q k1 + k2]| <= sum_m1 |Z[m1][k2]| <= P sqrt(2) max component, and
-// writes e to out_exp.
+// stores each column's largest scratch component of the workgroup to
+// colpart[m1][col] (float bits); colmax_reduce_kernel folds them into
+// colmax[col]; phase B stores X * 2^e per column with
+// e = pick_exp(1.5 P colmax), since |X[Q k1 + k2]| <= sum_m1 |Z[m1][k2]|
+// <= P sqrt(2) max component, and writes e to out_exp.
 #pragma once
 
 #include "fft_kernels.hpp"
@@ -98,9 +98,20 @@ tile_kernel(ColArgs a)
     const blk3 bk = xcd_block();       // neighbouring strips of one sub share an XCD
     const int sub = bk.y;              // m1 (MODE 1) or k2 (MODE 2)
     const int strip = bk.x;
-    const int col0 = strip * W;
-    const bool valid = w < a.ncols - col0;
-    const long long sbase = (long long)strip * a.P * a.Q * W;
+    // phase B writes rows Q*k1 + k2; Q is a multiple of the elements per
+    // 128-byte line, so all of them share one line offset.  Shifting the
+    // tile's columns left by that offset makes every row segment whole lines
+    // (no line is shared with the neighbouring tile); the shifted tiles still
+    // partition the columns, the first `skew` of them read from the previous
+    // strip's scratch block.
+    constexpr int ELINE = 128 / (2 * (int)sizeof(SS));
+    int skew = 0;
+    if constexpr (MODE == 2) skew = (int)(((long long)a.out_line_off + (long long)sub * a.pitch) % ELINE);
+    const int col0 = strip * W - skew;   // first column of the tile
+    if (col0 >= a.ncols) return;         // a trailing phase-B tile (uniform across the workgroup)
+    const int c = col0 + w;              // this thread's column
+    const bool valid = c >= 0 && c < a.ncols;
+    const long long sbase = (long long)strip * a.P * a.Q * W;   // phase A's scratch block
     const cpx<T> *twL = reinterpret_cast<const cpx<T> *>(a.tw_L);
 
     // ---- load: element e = jj + r*JPC of column w (stage-0 butterfly jj)
@@ -110,10 +121,14 @@ tile_kernel(ColArgs a)
     {
         const int e = jj + r * JPC;
         long long idx;
-        if constexpr (MODE == 1) idx = (long long)(sub + a.P * e) * a.pitch + col0 + w;
-        else idx = sbase + ((long long)sub * a.P + e) * W + w;
+        if constexpr (MODE == 1) idx = (long long)(sub + a.P * e) * a.pitch + c;
+        else
+        {
+            const int blk = valid ? c / W : 0;   // scratch block holding column c
+            idx = (((long long)blk * a.Q + sub) * a.P + e) * W + (c - blk * W);
+        }
         cpx<T> z = {T(0), T(0)};
-        if (MODE == 2 || valid)
+        if (valid)
         {
             z = stor<SIN>::ld(reinterpret_cast<const SIN *>(a.in) + 2 * idx);
             if constexpr (S16 && MODE == 1)
@@ -211,22 +226,22 @@ tile_kernel(ColArgs a)
     }
     if constexpr (S16 && MODE == 1)
     {
-        // this thread's column w: largest scratch component -> LDS -> colmax
+        // this thread's column w: largest scratch component -> LDS -> colpart[m1][col]
         float m = 0.0f;
 #pragma unroll
         for (int x = 0; x < 16; ++x) m = fmaxf(m, fmaxf(fabsf((float)v[x].x), fabsf((float)v[x].y)));
         atomicMax(&smax[w], __float_as_uint(m));
         __syncthreads();
-        if (t < W && t < a.ncols - col0 && a.colmax) atomicMax(&a.colmax[col0 + t], smax[t]);
+        if (t < W && t < a.ncols - col0 && a.colpart) a.colpart[(long long)sub * a.ncols + col0 + t] = smax[t];
     }
     T osc = T(1);
     if constexpr (S16 && MODE == 2)
     {
         if (valid)
         {
-            const int ex = pick_exp(__uint_as_float(a.colmax[col0 + w]) * 1.5f * (float)a.P);
+            const int ex = pick_exp(__uint_as_float(a.colmax[c]) * 1.5f * (float)a.P);
             osc = (T)ldexpf(1.0f, ex);
-            if (jj == 0 && a.out_exp) a.out_exp[col0 + w] = ex;   // the thread holding e = 0
+            if (jj == 0 && a.out_exp) a.out_exp[c] = ex;   // the thread holding e = 0
         }
     }
 #pragma unroll
@@ -247,8 +262,7 @@ tile_kernel(ColArgs a)
                 if constexpr (CONJ) z = conjc(z);
                 if constexpr (S16) z = scal(z, osc);
                 if (valid)
-                    stor<SOUT>::st(reinterpret_cast<SOUT *>(a.out) +
-                                       2 * ((long long)(a.Q * e + sub) * a.pitch + col0 + w), z);
+                    stor<SOUT>::st(reinterpret_cast<SOUT *>(a.out) + 2 * ((long long)(a.Q * e + sub) * a.pitch + c), z);
             }
         }
     }

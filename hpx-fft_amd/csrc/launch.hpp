@@ -163,11 +163,7 @@ hipError_t launch_col(const ColArgs &a, unsigned gx, unsigned gy, unsigned gz, h
     }
     else
     {
-        // MODE 1 writes the compute-precision scratch, MODE 2 reads it
-        using SI = typename std::conditional<MODE == 2, T, S>::type;
-        using SO = typename std::conditional<MODE == 1, T, S>::type;
-        hipLaunchKernelGGL((col_kernel<T, SI, SO, S, LG, MODE, CONJ>), dim3(gx, gy, gz),
-                           dim3(cfg<T, S>::template nt<MODE>(LG)), 0, s, a);
+        hipLaunchKernelGGL((col_kernel<T, S, LG, CONJ>), dim3(gx, gy, gz), dim3(cfg<T, S>::col_NT(LG)), 0, s, a);
     }
     return hipGetLastError();
 }
