@@ -21,10 +21,14 @@
 
 namespace hpxfft_dev {
 
-template <typename T, int L> struct bigrow
+// NB radix-16 butterflies per thread per stage: L/(16 NB) threads with 16 NB
+// points each (NB = 2 halves the workgroup of the longest FP32 rows so two
+// fit on a CU).  Butterfly j = t + b NT of a stage reads d[j + r M], M = L/16.
+template <typename T, int L, int NB = 1> struct bigrow
 {
     static constexpr int LG = ilog2c(L);
-    static constexpr int NT = L / 16;
+    static constexpr int M = L / 16;                   // radix-16 butterflies per stage
+    static constexpr int NT = M / NB;
     static constexpr int NR16 = LG / 4;                // radix-16 stages before the fused one
     static constexpr int RL = 1 << (LG % 4);           // radix of the fused final stage
     static constexpr int NSL = L / RL;                 // its Ns
@@ -35,8 +39,15 @@ template <typename T, int L> struct bigrow
     static constexpr bool SPLIT = (long)LP * 2 * (long)sizeof(T) > 40 * 1024;
     static constexpr int ns(int s) { return s == 0 ? 1 : 16 * ns(s - 1); }
     __device__ static __forceinline__ int pad(int e) { return e + (e >> 4); }
-    static_assert(LG % 4 != 0 && NR16 >= 2 && NMAIN >= 1 && HALF % NT == 0, "supported: L = 2^11, 2^13, 2^14");
+    static_assert(LG % 4 != 0 && NR16 >= 2 && NMAIN >= 1 && HALF % NT == 0 && M % 16 == 0,
+                  "supported: L = 2^11, 2^13, 2^14");
 };
+
+// butterflies per thread: two for FP32 (and 16-bit storage) rows of 2^14 points,
+// whose 512-thread workgroups then need <= 128 VGPRs (4 waves per SIMD) to
+// run two per CU
+template <typename T, int LG> constexpr int big_nb() { return sizeof(T) == 4 && LG == 14 ? 2 : 1; }
+template <typename T, int LG> constexpr int big_min_waves() { return big_nb<T, LG>() == 2 ? 4 : 1; }
 
 // stage twiddles w_{16 NS}^{r k}, r = 1..15, from the FP64 w_{2L} table:
 // table values w, w^2, w^4, w^8 and products of at most 3 factors, formed in
@@ -54,73 +65,94 @@ __device__ __forceinline__ void twiddle16(cpx<T> *v, const cpx<double> *__restri
     for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], cpx<T>{(T)ws[r].x, (T)ws[r].y});
 }
 
-template <typename T, int L, int S>
+template <typename T, int L, int NB, int S>
 __device__ __forceinline__ void radix16_stages(cpx<T> *v, T *lds, const cpx<double> *__restrict__ tw64, int t)
 {
-    using B = bigrow<T, L>;
+    using B = bigrow<T, L, NB>;
     // stages 0 .. NR16-2 (each followed by an exchange); stage NR16-1 is
     // computed by the caller and handed to the fused stage
     if constexpr (S + 1 < B::NR16)
     {
         constexpr int NS = B::ns(S);
-        const int k = t & (NS - 1);
-        if constexpr (NS > 1) twiddle16<T, L, NS>(v, tw64, k);
-        dft16(v);
+        // butterfly j: outputs e_q = (j/NS) 16 NS + k + q NS -> padded base + q*WS; next inputs j + r M
+        constexpr int WS = NS == 1 ? 1 : NS + NS / 16;
+        constexpr int RS = B::M + B::M / 16;
+        int wb[NB], rb[NB];
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
         {
-            // outputs e_q = (t/NS) 16 NS + k + q NS -> padded base + q*WS ; inputs t + r NT
-            constexpr int WS = NS == 1 ? 1 : NS + NS / 16;
-            constexpr int RS = B::NT + B::NT / 16;
-            const int wb = B::pad((t / NS) * 16 * NS + k), rb = B::pad(t);
-            if constexpr (!B::SPLIT)
-            {
-                cpx<T> *l = reinterpret_cast<cpx<T> *>(lds);
-#pragma unroll
-                for (int q = 0; q < 16; ++q) l[wb + q * WS] = v[q];
-                __syncthreads();
-#pragma unroll
-                for (int r = 0; r < 16; ++r) v[r] = l[rb + r * RS];
-                __syncthreads();
-            }
-            else
-            {
-#pragma unroll
-                for (int q = 0; q < 16; ++q) lds[wb + q * WS] = v[q].x;
-                __syncthreads();
-#pragma unroll
-                for (int r = 0; r < 16; ++r) v[r].x = lds[rb + r * RS];
-                __syncthreads();
-#pragma unroll
-                for (int q = 0; q < 16; ++q) lds[wb + q * WS] = v[q].y;
-                __syncthreads();
-#pragma unroll
-                for (int r = 0; r < 16; ++r) v[r].y = lds[rb + r * RS];
-                __syncthreads();
-            }
-            radix16_stages<T, L, S + 1>(v, lds, tw64, t);
+            const int j = t + b * B::NT, k = j & (NS - 1);
+            if constexpr (NS > 1) twiddle16<T, L, NS>(v + 16 * b, tw64, k);
+            dft16(v + 16 * b);
+            wb[b] = B::pad((j / NS) * 16 * NS + k);
+            rb[b] = B::pad(j);
         }
+        if constexpr (!B::SPLIT)
+        {
+            cpx<T> *l = reinterpret_cast<cpx<T> *>(lds);
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) l[wb[b] + q * WS] = v[16 * b + q];
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[16 * b + r] = l[rb[b] + r * RS];
+            __syncthreads();
+        }
+        else
+        {
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) lds[wb[b] + q * WS] = v[16 * b + q].x;
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[16 * b + r].x = lds[rb[b] + r * RS];
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int q = 0; q < 16; ++q) lds[wb[b] + q * WS] = v[16 * b + q].y;
+            __syncthreads();
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) v[16 * b + r].y = lds[rb[b] + r * RS];
+            __syncthreads();
+        }
+        radix16_stages<T, L, NB, S + 1>(v, lds, tw64, t);
     }
 }
 
 // gather the fused-stage inputs: for task j (butterflies j and NSL-j),
 // a[r] = d[j + r NSL], c[r] = d[NSL-j + r NSL]
-template <typename T, int L, bool HALFSEL>
-__device__ __forceinline__ void gather_fused(T *lds, const cpx<T> *v, cpx<T> (&a)[bigrow<T, L>::NMAIN][bigrow<T, L>::RL],
-                                             cpx<T> (&c)[bigrow<T, L>::NMAIN][bigrow<T, L>::RL],
-                                             cpx<T> (&s)[bigrow<T, L>::RL], int t)
+template <typename T, int L, int NB, bool HALFSEL>
+__device__ __forceinline__ void gather_fused(T *lds, const cpx<T> *v,
+                                             cpx<T> (&a)[bigrow<T, L, NB>::NMAIN][bigrow<T, L, NB>::RL],
+                                             cpx<T> (&c)[bigrow<T, L, NB>::NMAIN][bigrow<T, L, NB>::RL],
+                                             cpx<T> (&s)[bigrow<T, L, NB>::RL], int t)
 {
-    using B = bigrow<T, L>;
+    using B = bigrow<T, L, NB>;
     constexpr int RL = B::RL, NSL = B::NSL, NT = B::NT;
     constexpr int NS = B::ns(B::NR16 - 1);
     constexpr int WS = NS == 1 ? 1 : NS + NS / 16;
     constexpr int RS = NSL + NSL / 16;
-    const int k = t & (NS - 1);
-    const int wb = B::pad((t / NS) * 16 * NS + k);
     // write this thread's last radix-16 outputs (one component when split)
 #pragma unroll
-    for (int q = 0; q < 16; ++q)
+    for (int b = 0; b < NB; ++b)
     {
-        if constexpr (!B::SPLIT) reinterpret_cast<cpx<T> *>(lds)[wb + q * WS] = v[q];
-        else lds[wb + q * WS] = HALFSEL ? v[q].y : v[q].x;
+        const int j = t + b * NT, k = j & (NS - 1);
+        const int wb = B::pad((j / NS) * 16 * NS + k);
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+        {
+            if constexpr (!B::SPLIT) reinterpret_cast<cpx<T> *>(lds)[wb + q * WS] = v[16 * b + q];
+            else lds[wb + q * WS] = HALFSEL ? v[16 * b + q].y : v[16 * b + q].x;
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -192,29 +224,33 @@ __device__ __forceinline__ float cmax(cpx<T> a, cpx<T> b)
     return (float)fmax(fmax(fabs(a.x), fabs(a.y)), fmax(fabs(b.x), fabs(b.y)));
 }
 
-// Transform one row held in v[] (stage-0 inputs: v[r] = z[t + r NT]) and
-// store its Ny/2+1 outputs to orow (storage SO).  16-bit SO: smax is a
+// Transform one row held in v[] (stage-0 inputs: v[16 b + r] = z[t + b NT + r M])
+// and store its Ny/2+1 outputs to orow (storage SO).  16-bit SO: smax is a
 // workgroup slot cleared before the first barrier, oexp the row's exponent.
-template <typename T, typename SO, int LG>
+template <typename T, typename SO, int LG, int NB>
 __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__restrict__ tw,
                                              const cpx<double> *__restrict__ tw64, SO *orow, int t,
                                              unsigned *smax, int *oexp)
 {
     constexpr int L = 1 << LG;
-    using B = bigrow<T, L>;
+    using B = bigrow<T, L, NB>;
     constexpr int NT = B::NT, RL = B::RL, NSL = B::NSL, HALF = B::HALF;
 
-    radix16_stages<T, L, 0>(v, lds, tw64, t);
+    radix16_stages<T, L, NB, 0>(v, lds, tw64, t);
     {
         // last radix-16 stage; its outputs are handed to the fused stage through LDS
         constexpr int NS = B::ns(B::NR16 - 1);
-        const int k = t & (NS - 1);
-        if constexpr (NS > 1) twiddle16<T, L, NS>(v, tw64, k);
-        dft16(v);
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+        {
+            const int k = (t + b * NT) & (NS - 1);
+            if constexpr (NS > 1) twiddle16<T, L, NS>(v + 16 * b, tw64, k);
+            dft16(v + 16 * b);
+        }
     }
     cpx<T> a[B::NMAIN][RL], c[B::NMAIN][RL], s[RL];
-    gather_fused<T, L, false>(lds, v, a, c, s, t);
-    if constexpr (B::SPLIT) gather_fused<T, L, true>(lds, v, a, c, s, t);
+    gather_fused<T, L, NB, false>(lds, v, a, c, s, t);
+    if constexpr (B::SPLIT) gather_fused<T, L, NB, true>(lds, v, a, c, s, t);
 
     const bool self = t == HALF % NT;
     if (self)
@@ -349,11 +385,12 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
 // Forward r2c of long rows: workgroup g transforms row g of the nrows x
 // batch rows.  SI / SO: input / output storage (T, or 16-bit with T = float).
 template <typename T, typename SI, typename SO, int LG>
-__global__ void __launch_bounds__((1 << LG) / 16)
+__global__ void __launch_bounds__((1 << LG) / (16 * big_nb<T, LG>()), (big_min_waves<T, LG>()))
 row_r2c_big_kernel(RowArgs a)
 {
     constexpr int L = 1 << LG;
-    using B = bigrow<T, L>;
+    constexpr int NB = big_nb<T, LG>();
+    using B = bigrow<T, L, NB>;
     constexpr int NT = B::NT;
     __shared__ T lds[B::SPLIT ? B::LP : 2 * B::LP];
     __shared__ unsigned smax;
@@ -362,21 +399,21 @@ row_r2c_big_kernel(RowArgs a)
     const int g = xcd_order((int)blockIdx.x, (int)gridDim.x);   // consecutive rows share an XCD
     const int b = g / a.nrows, row = g - b * a.nrows;
     const SI *irow = reinterpret_cast<const SI *>(a.in) + (long long)b * a.dist_in + (long long)row * a.ld_in;
-    cpx<T> v[16];
+    cpx<T> v[16 * NB];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = stor<SI>::ld(irow + 2 * (t + r * NT));
+    for (int u = 0; u < 16 * NB; ++u) v[u] = stor<SI>::ld(irow + 2 * (t + (u / 16) * NT + (u % 16) * B::M));
     if constexpr (is16<SI>())
     {
         if (a.in_exp)
         {
             const T sc = (T)ldexpf(1.0f, -a.in_exp[(long long)b * a.exp_dist + row]);
 #pragma unroll
-            for (int r = 0; r < 16; ++r) v[r] = scal(v[r], sc);
+            for (int u = 0; u < 16 * NB; ++u) v[u] = scal(v[u], sc);
         }
     }
     SO *orow = reinterpret_cast<SO *>(a.out) + (long long)b * a.dist_out + (long long)row * a.ld_out;
     int *oexp = (is16<SO>() && a.out_exp) ? a.out_exp + (long long)b * a.exp_dist + row : nullptr;
-    big_row_body<T, SO, LG>(v, lds, reinterpret_cast<const cpx<T> *>(a.tw),
+    big_row_body<T, SO, LG, NB>(v, lds, reinterpret_cast<const cpx<T> *>(a.tw),
                             reinterpret_cast<const cpx<double> *>(a.tw64), orow, t, &smax, oexp);
 }
 

@@ -68,13 +68,20 @@ template <typename T, int L, int W> struct tileinfo
     static constexpr int NT = W * L / 16;
     static constexpr int JPC = L / 16;       // threads per column
     static constexpr int NB1 = 16 / R1;      // second-stage butterflies per thread
-    // FP64 tiles, and tiles whose complex exchange would exceed 64 KiB, exchange
-    // real and imaginary halves (half the LDS, twice the workgroups per CU)
+    // FP64 tiles, and tiles whose complex exchange would exceed 64 KiB,
+    // exchange real and imaginary halves (half the LDS, twice the workgroups
+    // per CU where registers allow)
     static constexpr bool SPLIT = sizeof(T) == 8 || (long)L * W * 2 * (long)sizeof(T) > 64 * 1024;
 };
 
+// 16-bit phase A at 256 rows: 1024 threads; at <= 64 VGPRs two workgroups share a CU
+template <typename T, typename SS, int LG, int MODE> constexpr int tile_min_waves()
+{
+    return (sizeof(SS) == 2 && LG == 8 && MODE == 1) ? 8 : 1;
+}
+
 template <typename T, typename SS, int LG, int MODE, bool CONJ>
-__global__ void __launch_bounds__((cfg<T, SS>::TW << LG) / 16)
+__global__ void __launch_bounds__((cfg<T, SS>::TW << LG) / 16, (tile_min_waves<T, SS, LG, MODE>()))
 tile_kernel(ColArgs a)
 {
     constexpr int L = 1 << LG;

@@ -133,7 +133,8 @@ hipError_t launch_row_r2c(const RowArgs &a, unsigned gx, unsigned gy, hipStream_
     {
         (void)gx;
         const unsigned total = (unsigned)a.nrows * gy;
-        hipLaunchKernelGGL((row_r2c_big_kernel<T, S, S, LG>), dim3(total), dim3((1 << LG) / 16), 0, s, a);
+        hipLaunchKernelGGL((row_r2c_big_kernel<T, S, S, LG>), dim3(total), dim3((1 << LG) / (16 * big_nb<T, LG>())), 0,
+                           s, a);
     }
     else
     {
@@ -226,7 +227,10 @@ template <typename T, typename S> geom row_geom_t(int lg)
 {
     using C = cfg<T, S>;
     if (lg == 11 || lg == 13 || lg == 14)
-        return geom{(1 << lg) / 16, 1, 16, (long)((1 << lg) + (1 << lg) / 16) * 2 * (long)sizeof(T) > 40 * 1024};
+    {
+        const int nb = sizeof(T) == 4 && lg == 14 ? 2 : 1;   // as big_nb<T, LG>()
+        return geom{(1 << lg) / (16 * nb), 1, 16 * nb, (long)((1 << lg) + (1 << lg) / 16) * 2 * (long)sizeof(T) > 40 * 1024};
+    }
     return geom{C::row_NT(lg), C::row_NSEQ(lg), C::row_E(lg), C::row_SPLIT(lg)};
 }
 template <typename T, typename S> geom col_geom_t(int lg)
