@@ -315,6 +315,7 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
     a.P = P;
     a.Q = Q;
     a.tw_N = tw_ptr(p, p->nx);
+    a.tw64_N = tw64_ptr(p, p->nx);
     a.colpart = p->colpart;
     a.colmax = p->colmax;
     const unsigned gx = (unsigned)((p->ncols + W - 1) / W);
@@ -330,6 +331,7 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         ab.in = in_b;
         ab.out = p->scratch;
         ab.tw_L = tw_ptr(p, (size_t)Q);
+        ab.tw64_L = tw64_ptr(p, (size_t)Q);
         HIPCHK(col(p->prec, p->lgQ, 1, conj, ab, gx, (unsigned)P, 1, p->stream));
         if (p->colpart)
         {
@@ -346,6 +348,7 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         ab.out = out_b;
         ab.out_line_off = (int)((reinterpret_cast<uintptr_t>(out_b) / (2 * sb)) % (uintptr_t)eline);
         ab.tw_L = tw_ptr(p, (size_t)P);
+        ab.tw64_L = tw64_ptr(p, (size_t)P);
         const unsigned gxb = (unsigned)((p->ncols + eline - 2) / W + 1);
         HIPCHK(col(p->prec, p->lgP, 2, conj, ab, gxb, (unsigned)Q, 1, p->stream));
     }

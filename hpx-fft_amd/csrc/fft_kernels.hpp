@@ -46,8 +46,8 @@ template <typename T, typename S> struct cfg
         // LDS layout [e][W]: padded only when a W-row is narrower than 128 B
         return (long)(W << lg) * 2 * (long)sizeof(T) > lds_budget;
     }
-    // four-step tiles: TW adjacent columns (512 B of FP64/FP32... storage per row
-    // segment for 8-byte elements, 256 B for 16-bit storage) x L rows
+    // four-step tiles: TW adjacent columns x L rows (row segments of 512 B for
+    // FP64 and FP32 storage, 256 B for 16-bit storage)
     static constexpr int TW = sizeof(S) == 8 ? 32 : 64;
     static constexpr int tile_E(int lg) { return 16; }
     static constexpr int tile_NT(int lg) { return (TW << lg) / 16; }
@@ -81,6 +81,8 @@ struct ColArgs
     int out_line_off;  // four-step phase B: (output address / complex size) mod elements per 128-byte line
     const void *tw_L;  // w_L^m, m < L (sub-FFT length)
     const void *tw_N;  // w_Nx^m, m < Nx (MODE 1 post-twiddle)
+    const void *tw64_L;  // the same tables in FP64 (FP32 four-step tiles form their twiddles in FP64)
+    const void *tw64_N;
     const int *row_exp;    // 16-bit inputs (MODE 0,1): per-row exponent of the rows
     unsigned *colpart;     // 16-bit four-step phase A: per-workgroup column maxima [m1][ncols] (float bits)
     unsigned *colmax;      // 16-bit four-step phase B: per-column maxima (reduced from colpart)

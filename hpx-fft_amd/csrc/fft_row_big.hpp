@@ -43,10 +43,10 @@ template <typename T, int L, int NB = 1> struct bigrow
                   "supported: L = 2^11, 2^13, 2^14");
 };
 
-// butterflies per thread: two for FP32 (and 16-bit storage) rows of 2^14 points,
-// whose 512-thread workgroups then need <= 128 VGPRs (4 waves per SIMD) to
-// run two per CU
-template <typename T, int LG> constexpr int big_nb() { return sizeof(T) == 4 && LG == 14 ? 2 : 1; }
+// butterflies per thread: two for FP32 (and 16-bit storage) rows of 2^13 and
+// 2^14 points, whose 256- / 512-thread workgroups then need <= 128 VGPRs
+// (4 waves per SIMD) to run four / two per CU
+template <typename T, int LG> constexpr int big_nb() { return sizeof(T) == 4 && LG >= 13 ? 2 : 1; }
 template <typename T, int LG> constexpr int big_min_waves() { return big_nb<T, LG>() == 2 ? 4 : 1; }
 
 // stage twiddles w_{16 NS}^{r k}, r = 1..15, from the FP64 w_{2L} table:
@@ -58,11 +58,23 @@ __device__ __forceinline__ void twiddle16(cpx<T> *v, const cpx<double> *__restri
     const int base = k * (L / (8 * NS));   // index of w_{16 NS}^k in the w_{2L} table
     const cpx<double> w1 = tw64[base], w2 = tw64[2 * base], w4 = tw64[4 * base], w8 = tw64[8 * base];
     const cpx<double> w3 = cmul(w2, w1), w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
-    const cpx<double> ws[16] = {{1.0, 0.0},      w1,           w2,           w3,           w4,           w5,
-                                w6,              w7,           w8,           cmul(w8, w1), cmul(w8, w2), cmul(w8, w3),
-                                cmul(w8, w4),    cmul(w8, w5), cmul(w8, w6), cmul(w8, w7)};
-#pragma unroll
-    for (int r = 1; r < 16; ++r) v[r] = cmul(v[r], cpx<T>{(T)ws[r].x, (T)ws[r].y});
+    // each factor is applied as soon as it is formed (fewer live FP64 registers)
+    auto app = [&](int r, const cpx<double> &w) { v[r] = cmul(v[r], cpx<T>{(T)w.x, (T)w.y}); };
+    app(1, w1);
+    app(2, w2);
+    app(3, w3);
+    app(4, w4);
+    app(5, w5);
+    app(6, w6);
+    app(7, w7);
+    app(8, w8);
+    app(9, cmul(w8, w1));
+    app(10, cmul(w8, w2));
+    app(11, cmul(w8, w3));
+    app(12, cmul(w8, w4));
+    app(13, cmul(w8, w5));
+    app(14, cmul(w8, w6));
+    app(15, cmul(w8, w7));
 }
 
 template <typename T, int L, int NB, int S>

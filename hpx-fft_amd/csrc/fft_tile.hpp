@@ -25,6 +25,43 @@
 
 namespace hpxfft_dev {
 
+// FP32 tiles: twiddles w^{r j}, r = 1..R-1, formed in FP64 from the FP64
+// table values w, w^2, w^4, w^8 (products of at most three) and rounded once
+template <typename T, int R>
+__device__ __forceinline__ void twiddle_r64(cpx<T> *a, const cpx<double> *__restrict__ tab, int jstep)
+{
+    auto app = [&](int r, const cpx<double> &w) { a[r] = cmul(a[r], cpx<T>{(T)w.x, (T)w.y}); };
+    const cpx<double> w1 = tab[jstep];
+    app(1, w1);
+    if constexpr (R >= 4)
+    {
+        const cpx<double> w2 = tab[2 * jstep], w3 = cmul(w2, w1);
+        app(2, w2);
+        app(3, w3);
+        if constexpr (R >= 8)
+        {
+            const cpx<double> w4 = tab[4 * jstep];
+            const cpx<double> w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
+            app(4, w4);
+            app(5, w5);
+            app(6, w6);
+            app(7, w7);
+            if constexpr (R == 16)
+            {
+                const cpx<double> w8 = tab[8 * jstep];
+                app(8, w8);
+                app(9, cmul(w8, w1));
+                app(10, cmul(w8, w2));
+                app(11, cmul(w8, w3));
+                app(12, cmul(w8, w4));
+                app(13, cmul(w8, w5));
+                app(14, cmul(w8, w6));
+                app(15, cmul(w8, w7));
+            }
+        }
+    }
+}
+
 // twiddles w^{r j}, r = 1..R-1 (R <= 16) applied to a[1..R-1]; w^m = tab[m*step]
 template <typename T, int R>
 __device__ __forceinline__ void twiddle_r(cpx<T> *a, const cpx<T> *__restrict__ tab, int jstep)
@@ -68,20 +105,14 @@ template <typename T, int L, int W> struct tileinfo
     static constexpr int NT = W * L / 16;
     static constexpr int JPC = L / 16;       // threads per column
     static constexpr int NB1 = 16 / R1;      // second-stage butterflies per thread
-    // FP64 tiles, and tiles whose complex exchange would exceed 64 KiB,
-    // exchange real and imaginary halves (half the LDS, twice the workgroups
-    // per CU where registers allow)
-    static constexpr bool SPLIT = sizeof(T) == 8 || (long)L * W * 2 * (long)sizeof(T) > 64 * 1024;
+    // FP64 tiles, and tiles whose complex exchange would take 64 KiB or more,
+    // exchange real and imaginary halves (half the LDS, more workgroups per
+    // CU where registers allow)
+    static constexpr bool SPLIT = sizeof(T) == 8 || (long)L * W * 2 * (long)sizeof(T) >= 64 * 1024;
 };
 
-// 16-bit phase A at 256 rows: 1024 threads; at <= 64 VGPRs two workgroups share a CU
-template <typename T, typename SS, int LG, int MODE> constexpr int tile_min_waves()
-{
-    return (sizeof(SS) == 2 && LG == 8 && MODE == 1) ? 8 : 1;
-}
-
 template <typename T, typename SS, int LG, int MODE, bool CONJ>
-__global__ void __launch_bounds__((cfg<T, SS>::TW << LG) / 16, (tile_min_waves<T, SS, LG, MODE>()))
+__global__ void __launch_bounds__((cfg<T, SS>::TW << LG) / 16)
 tile_kernel(ColArgs a)
 {
     constexpr int L = 1 << LG;
@@ -192,7 +223,8 @@ tile_kernel(ColArgs a)
     for (int i = 0; i < NB1; ++i)
     {
         const int j1 = jj + i * JPC;
-        twiddle_r<T, R1>(v + i * R1, twL, j1);   // w_L^{r j1}
+        if constexpr (sizeof(T) == 8) twiddle_r<T, R1>(v + i * R1, twL, j1);   // w_L^{r j1}
+        else twiddle_r64<T, R1>(v + i * R1, reinterpret_cast<const cpx<double> *>(a.tw64_L), j1);
         dftR<T, R1>(v + i * R1);
     }
     // outputs: butterfly j1, q -> e = j1 + 16 q
@@ -221,12 +253,19 @@ tile_kernel(ColArgs a)
             }
             else
             {
+                // FP32: the same two factors from the FP64 table, multiplied in FP64, rounded once
+                const cpx<double> *t64 = reinterpret_cast<const cpx<double> *>(a.tw64_N);
 #pragma unroll
                 for (int i = 0; i < NB1; ++i)
                 {
                     const int j1 = jj + i * JPC;
+                    const cpx<double> wb = t64[sub * j1];
 #pragma unroll
-                    for (int q = 0; q < R1; ++q) v[i * R1 + q] = cmul(v[i * R1 + q], twN[sub * (j1 + 16 * q)]);
+                    for (int q = 0; q < R1; ++q)
+                    {
+                        const cpx<double> w = q == 0 ? wb : cmul(wb, t64[(16 * sub * q) & nmask]);
+                        v[i * R1 + q] = cmul(v[i * R1 + q], cpx<T>{(T)w.x, (T)w.y});
+                    }
                 }
             }
         }

@@ -228,7 +228,7 @@ template <typename T, typename S> geom row_geom_t(int lg)
     using C = cfg<T, S>;
     if (lg == 11 || lg == 13 || lg == 14)
     {
-        const int nb = sizeof(T) == 4 && lg == 14 ? 2 : 1;   // as big_nb<T, LG>()
+        const int nb = sizeof(T) == 4 && lg >= 13 ? 2 : 1;   // as big_nb<T, LG>()
         return geom{(1 << lg) / (16 * nb), 1, 16 * nb, (long)((1 << lg) + (1 << lg) / 16) * 2 * (long)sizeof(T) > 40 * 1024};
     }
     return geom{C::row_NT(lg), C::row_NSEQ(lg), C::row_E(lg), C::row_SPLIT(lg)};
