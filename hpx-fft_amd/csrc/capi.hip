@@ -118,10 +118,16 @@ int get_tables(int dev, dev_tables &out)
         }
     }
     dev_tables t;
-    HIPCHK(hipMalloc(&t.tw64, sizeof(double) * 2 * n));
-    HIPCHK(hipMalloc(&t.tw32, sizeof(float) * 2 * n));
-    HIPCHK(hipMemcpy(t.tw64, h64.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(t.tw32, h32.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+    hipError_t e = hipMalloc(&t.tw64, sizeof(double) * 2 * n);
+    if (e == hipSuccess) e = hipMalloc(&t.tw32, sizeof(float) * 2 * n);
+    if (e == hipSuccess) e = hipMemcpy(t.tw64, h64.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.tw32, h32.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+    {
+        (void)hipFree(t.tw64);
+        (void)hipFree(t.tw32);
+        HIPCHK(e);
+    }
     g_tables[dev] = t;
     out = t;
     return HPXFFT_OK;
@@ -336,7 +342,7 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         if (p->colpart)
         {
             // 16-bit: fold phase A's per-workgroup maxima into one per column (every entry is rewritten)
-            hipLaunchKernelGGL(hpxfft_dev::colmax_reduce_kernel<0>, dim3((unsigned)((p->ncols + 255) / 256)), dim3(256),
+            hipLaunchKernelGGL(hpxfft_dev::colmax_reduce_kernel<0>, dim3((unsigned)((p->ncols + 63) / 64)), dim3(256),
                                0, p->stream, p->colpart, p->colmax, (int)p->ncols, P);
             HIPCHK(hipGetLastError());
         }

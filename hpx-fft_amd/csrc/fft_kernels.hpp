@@ -491,14 +491,20 @@ template <int UNUSED = 0> __global__ void zero_u32_kernel(unsigned *p, long long
 }
 
 // colmax[c] = max over m1 < P of colpart[m1 * ncols + c] (non-negative float
-// bit patterns order as unsigned integers)
-template <int UNUSED = 0> __global__ void colmax_reduce_kernel(const unsigned *colpart, unsigned *colmax, int ncols, int P)
+// bit patterns order as unsigned integers).  256 threads: 64 columns x 4
+// slices of the P partial rows, combined through LDS.
+template <int UNUSED = 0> __global__ void __launch_bounds__(256)
+colmax_reduce_kernel(const unsigned *colpart, unsigned *colmax, int ncols, int P)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncols) return;
+    __shared__ unsigned part[4][64];
+    const int lc = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + lc;
     unsigned m = 0u;
-    for (int i = 0; i < P; ++i) m = max(m, colpart[(long long)i * ncols + c]);
-    colmax[c] = m;
+    if (c < ncols)
+        for (int i = sl; i < P; i += 4) m = max(m, colpart[(long long)i * ncols + c]);
+    part[sl][lc] = m;
+    __syncthreads();
+    if (sl == 0 && c < ncols) colmax[c] = max(max(part[0][lc], part[1][lc]), max(part[2][lc], part[3][lc]));
 }
 
 }   // namespace hpxfft_dev

@@ -223,7 +223,8 @@ tile_kernel(ColArgs a)
     for (int i = 0; i < NB1; ++i)
     {
         const int j1 = jj + i * JPC;
-        if constexpr (sizeof(T) == 8) twiddle_r<T, R1>(v + i * R1, twL, j1);   // w_L^{r j1}
+        // w_L^{r j1}: FP64 and 16-bit storage from their own tables, FP32 formed in FP64
+        if constexpr (sizeof(T) == 8 || S16) twiddle_r<T, R1>(v + i * R1, twL, j1);
         else twiddle_r64<T, R1>(v + i * R1, reinterpret_cast<const cpx<double> *>(a.tw64_L), j1);
         dftR<T, R1>(v + i * R1);
     }
@@ -235,9 +236,10 @@ tile_kernel(ColArgs a)
         {
             const cpx<T> *twN = reinterpret_cast<const cpx<T> *>(a.tw_N);
             const int nmask = a.P * a.Q - 1;
-            if constexpr (sizeof(T) == 8)
+            if constexpr (sizeof(T) == 8 || S16)
             {
-                // FP64: w^{m1 j1} (per lane) * w^{16 m1 q} (uniform across the workgroup), both table values
+                // w^{m1 j1} (per lane) * w^{16 m1 q} (uniform across the workgroup), both table values
+                // (FP64, or the FP32 table for 16-bit storage)
 #pragma unroll
                 for (int i = 0; i < NB1; ++i)
                 {
