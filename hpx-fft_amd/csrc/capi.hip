@@ -324,40 +324,32 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
     a.tw64_N = tw64_ptr(p, p->nx);
     a.colpart = p->colpart;
     a.colmax = p->colmax;
-    const unsigned gx = (unsigned)((p->ncols + W - 1) / W);
-    for (size_t b = 0; b < p->batch; ++b)
+    a.sdist = (long long)(p->nx * p->CW);
+    const unsigned gx = (unsigned)((p->ncols + W - 1) / W), nb = (unsigned)p->batch;
+    // phase A: spectrum -> scratch (one scratch slice per grid: all grids in one launch)
+    a.in = spec_in;
+    a.out = p->scratch;
+    a.tw_L = tw_ptr(p, (size_t)Q);
+    a.tw64_L = tw64_ptr(p, (size_t)Q);
+    HIPCHK(col(p->prec, p->lgQ, 1, conj, a, gx, (unsigned)P, nb, p->stream));
+    if (p->colpart)
     {
-        const char *in_b = reinterpret_cast<const char *>(spec_in) + b * p->dist_out * 2 * sb;
-        char *out_b = reinterpret_cast<char *>(spec_out) + b * p->dist_out * 2 * sb;
-        ColArgs ab = a;
-        ab.dist = 0;
-        if (ab.row_exp) ab.row_exp += b * p->nx;
-        if (ab.out_exp) ab.out_exp += b * p->ncols;
-        // phase A: spectrum -> scratch
-        ab.in = in_b;
-        ab.out = p->scratch;
-        ab.tw_L = tw_ptr(p, (size_t)Q);
-        ab.tw64_L = tw64_ptr(p, (size_t)Q);
-        HIPCHK(col(p->prec, p->lgQ, 1, conj, ab, gx, (unsigned)P, 1, p->stream));
-        if (p->colpart)
-        {
-            // 16-bit: fold phase A's per-workgroup maxima into one per column (every entry is rewritten)
-            hipLaunchKernelGGL(hpxfft_dev::colmax_reduce_kernel<0>, dim3((unsigned)((p->ncols + 63) / 64)), dim3(256),
-                               0, p->stream, p->colpart, p->colmax, (int)p->ncols, P);
-            HIPCHK(hipGetLastError());
-        }
-        if (mid && p->batch == 1) HIPCHK(hipEventRecord(mid, p->stream));
-        // phase B: scratch -> spectrum; its tiles are shifted to whole 128-byte
-        // lines of the output rows, which can take one more tile per row
-        const int eline = (int)(128 / (2 * sb));
-        ab.in = p->scratch;
-        ab.out = out_b;
-        ab.out_line_off = (int)((reinterpret_cast<uintptr_t>(out_b) / (2 * sb)) % (uintptr_t)eline);
-        ab.tw_L = tw_ptr(p, (size_t)P);
-        ab.tw64_L = tw64_ptr(p, (size_t)P);
-        const unsigned gxb = (unsigned)((p->ncols + eline - 2) / W + 1);
-        HIPCHK(col(p->prec, p->lgP, 2, conj, ab, gxb, (unsigned)Q, 1, p->stream));
+        // 16-bit: fold phase A's per-workgroup maxima into one per column (every entry is rewritten)
+        hipLaunchKernelGGL(hpxfft_dev::colmax_reduce_kernel<0>, dim3((unsigned)((p->ncols + 63) / 64), nb), dim3(256),
+                           0, p->stream, p->colpart, p->colmax, (int)p->ncols, P);
+        HIPCHK(hipGetLastError());
     }
+    if (mid) HIPCHK(hipEventRecord(mid, p->stream));
+    // phase B: scratch -> spectrum; its tiles are shifted to whole 128-byte
+    // lines of the output rows, which can take one more tile per row
+    const int eline = (int)(128 / (2 * sb));
+    a.in = p->scratch;
+    a.out = spec_out;
+    a.out_line_off = (int)((reinterpret_cast<uintptr_t>(spec_out) / (2 * sb)) % (uintptr_t)eline);
+    a.tw_L = tw_ptr(p, (size_t)P);
+    a.tw64_L = tw64_ptr(p, (size_t)P);
+    const unsigned gxb = (unsigned)((p->ncols + eline - 2) / W + 1);
+    HIPCHK(col(p->prec, p->lgP, 2, conj, a, gxb, (unsigned)Q, nb, p->stream));
     return HPXFFT_OK;
 }
 
@@ -391,7 +383,7 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     if (int rc = run_columns(p, d_out, d_out, false, s16 ? col_exp : nullptr, p->ev[3])) return rc;
     HIPCHK(hipEventRecord(p->ev[2], p->stream));
     p->ran = true;
-    p->col_split = p->four_step && p->batch == 1;
+    p->col_split = p->four_step;
     p->h2d = p->d2h = 0.0;
     return HPXFFT_OK;
 }
@@ -425,11 +417,11 @@ int plan_common(hpxfft_plan_s *p)
         if (p->lgP < COL_TILE_MIN_LG) { p->lgP = COL_TILE_MIN_LG; p->lgQ = p->lgx - p->lgP; }
         if (p->lgQ > COL_TILE_MAX_LG || p->lgP > COL_TILE_MAX_LG)
             return fail(HPXFFT_UNSUPPORTED_SIZE, "Nx too large");
-        // scratch: one P x Q x TW block per strip of TW columns (one grid at a time)
+        // scratch: one P x Q x TW block per strip of TW columns, one slice per grid
         const size_t cb = compute_bytes(p->prec) * 2;
         const size_t tw_cols = (size_t)tile_geom(p->prec, p->lgQ).nseq;
         p->CW = (p->ncols + tw_cols - 1) / tw_cols * tw_cols;
-        p->scratch_bytes = p->nx * p->CW * cb;
+        p->scratch_bytes = p->nx * p->CW * cb * p->batch;
         HIPCHK(hipMalloc(&p->scratch, p->scratch_bytes));
     }
     if (p->prec == P_F16 || p->prec == P_BF16)
@@ -438,9 +430,8 @@ int plan_common(hpxfft_plan_s *p)
         HIPCHK(hipMalloc(&p->col_exp_own, sizeof(int) * p->ncols * p->batch));
         if (p->four_step)
         {
-            // reused by each grid of a batch in turn (the column pass runs grid by grid)
-            HIPCHK(hipMalloc(&p->colpart, sizeof(unsigned) * p->ncols * ((size_t)1 << p->lgP)));
-            HIPCHK(hipMalloc(&p->colmax, sizeof(unsigned) * p->ncols));
+            HIPCHK(hipMalloc(&p->colpart, sizeof(unsigned) * p->ncols * ((size_t)1 << p->lgP) * p->batch));
+            HIPCHK(hipMalloc(&p->colmax, sizeof(unsigned) * p->ncols * p->batch));
         }
     }
     (void)sb;

@@ -79,6 +79,7 @@ struct ColArgs
     int ncols;         // valid columns of the spectrum (Ny/2+1)
     int P, Q;          // four-step split Nx = P*Q; scratch index ((strip*Q + k2)*P + m1)*TW + w
     int out_line_off;  // four-step phase B: (output address / complex size) mod elements per 128-byte line
+    long long sdist;   // four-step: scratch complex elements per grid
     const void *tw_L;  // w_L^m, m < L (sub-FFT length)
     const void *tw_N;  // w_Nx^m, m < Nx (MODE 1 post-twiddle)
     const void *tw64_L;  // the same tables in FP64 (FP32 four-step tiles form their twiddles in FP64)
@@ -491,12 +492,14 @@ template <int UNUSED = 0> __global__ void zero_u32_kernel(unsigned *p, long long
 }
 
 // colmax[c] = max over m1 < P of colpart[m1 * ncols + c] (non-negative float
-// bit patterns order as unsigned integers).  256 threads: 64 columns x 4
-// slices of the P partial rows, combined through LDS.
+// bit patterns order as unsigned integers), per grid.  256 threads: 64
+// columns x 4 slices of the P partial rows, combined through LDS.
 template <int UNUSED = 0> __global__ void __launch_bounds__(256)
 colmax_reduce_kernel(const unsigned *colpart, unsigned *colmax, int ncols, int P)
 {
     __shared__ unsigned part[4][64];
+    colpart += (long long)blockIdx.y * P * ncols;   // grid blockIdx.y of a batch
+    colmax += (long long)blockIdx.y * ncols;
     const int lc = threadIdx.x & 63, sl = threadIdx.x >> 6;
     const int c = blockIdx.x * 64 + lc;
     unsigned m = 0u;

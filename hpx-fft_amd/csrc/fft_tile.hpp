@@ -136,6 +136,10 @@ tile_kernel(ColArgs a)
     const blk3 bk = xcd_block();       // neighbouring strips of one sub share an XCD
     const int sub = bk.y;              // m1 (MODE 1) or k2 (MODE 2)
     const int strip = bk.x;
+    const long long gb = bk.z;         // grid of a batch
+    // this grid's spectrum, scratch slice and exponent / maximum arrays
+    const SIN *gin = reinterpret_cast<const SIN *>(a.in) + 2 * gb * (MODE == 1 ? a.dist : a.sdist);
+    SOUT *gout = reinterpret_cast<SOUT *>(a.out) + 2 * gb * (MODE == 1 ? a.sdist : a.dist);
     // phase B writes rows Q*k1 + k2; Q is a multiple of the elements per
     // 128-byte line, so all of them share one line offset.  Shifting the
     // tile's columns left by that offset makes every row segment whole lines
@@ -144,12 +148,12 @@ tile_kernel(ColArgs a)
     // strip's scratch block.
     constexpr int ELINE = 128 / (2 * (int)sizeof(SS));
     int skew = 0;
-    if constexpr (MODE == 2) skew = (int)(((long long)a.out_line_off + (long long)sub * a.pitch) % ELINE);
+    if constexpr (MODE == 2) skew = (int)((a.out_line_off + gb * a.dist + (long long)sub * a.pitch) % ELINE);
     const int col0 = strip * W - skew;   // first column of the tile
     if (col0 >= a.ncols) return;         // a trailing phase-B tile (uniform across the workgroup)
     const int c = col0 + w;              // this thread's column
     const bool valid = c >= 0 && c < a.ncols;
-    const long long sbase = (long long)strip * a.P * a.Q * W;   // phase A's scratch block
+    const long long sbase = (long long)strip * a.P * a.Q * W;   // phase A's scratch block (within the slice)
     const cpx<T> *twL = reinterpret_cast<const cpx<T> *>(a.tw_L);
 
     // ---- load: element e = jj + r*JPC of column w (stage-0 butterfly jj)
@@ -168,10 +172,10 @@ tile_kernel(ColArgs a)
         cpx<T> z = {T(0), T(0)};
         if (valid)
         {
-            z = stor<SIN>::ld(reinterpret_cast<const SIN *>(a.in) + 2 * idx);
+            z = stor<SIN>::ld(gin + 2 * idx);
             if constexpr (S16 && MODE == 1)
             {
-                if (a.row_exp) z = scal(z, (T)ldexpf(1.0f, -a.row_exp[sub + a.P * e]));
+                if (a.row_exp) z = scal(z, (T)ldexpf(1.0f, -a.row_exp[gb * a.rexp_dist + sub + a.P * e]));
             }
         }
         if constexpr (CONJ && MODE == 1) z = conjc(z);
@@ -280,16 +284,17 @@ tile_kernel(ColArgs a)
         for (int x = 0; x < 16; ++x) m = fmaxf(m, fmaxf(fabsf((float)v[x].x), fabsf((float)v[x].y)));
         atomicMax(&smax[w], __float_as_uint(m));
         __syncthreads();
-        if (t < W && t < a.ncols - col0 && a.colpart) a.colpart[(long long)sub * a.ncols + col0 + t] = smax[t];
+        if (t < W && t < a.ncols - col0 && a.colpart)
+            a.colpart[(gb * a.P + sub) * a.ncols + col0 + t] = smax[t];
     }
     T osc = T(1);
     if constexpr (S16 && MODE == 2)
     {
         if (valid)
         {
-            const int ex = pick_exp(__uint_as_float(a.colmax[c]) * 1.5f * (float)a.P);
+            const int ex = pick_exp(__uint_as_float(a.colmax[gb * a.ncols + c]) * 1.5f * (float)a.P);
             osc = (T)ldexpf(1.0f, ex);
-            if (jj == 0 && a.out_exp) a.out_exp[c] = ex;   // the thread holding e = 0
+            if (jj == 0 && a.out_exp) a.out_exp[gb * a.cexp_dist + c] = ex;   // the thread holding e = 0
         }
     }
 #pragma unroll
@@ -303,14 +308,13 @@ tile_kernel(ColArgs a)
             cpx<T> z = v[i * R1 + q];
             if constexpr (MODE == 1)
             {
-                stor<SOUT>::st(reinterpret_cast<SOUT *>(a.out) + 2 * (sbase + ((long long)e * a.P + sub) * W + w), z);
+                stor<SOUT>::st(gout + 2 * (sbase + ((long long)e * a.P + sub) * W + w), z);
             }
             else
             {
                 if constexpr (CONJ) z = conjc(z);
                 if constexpr (S16) z = scal(z, osc);
-                if (valid)
-                    stor<SOUT>::st(reinterpret_cast<SOUT *>(a.out) + 2 * ((long long)(a.Q * e + sub) * a.pitch + c), z);
+                if (valid) stor<SOUT>::st(gout + 2 * ((long long)(a.Q * e + sub) * a.pitch + c), z);
             }
         }
     }

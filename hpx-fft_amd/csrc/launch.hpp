@@ -157,10 +157,9 @@ hipError_t launch_col(const ColArgs &a, unsigned gx, unsigned gy, unsigned gz, h
 {
     if constexpr (MODE != 0)
     {
-        // four-step tiles: specialised two-stage kernel (scratch in the compute type)
-        (void)gz;
-        hipLaunchKernelGGL((tile_kernel<T, S, LG, MODE, CONJ>), dim3(gx, gy), dim3((cfg<T, S>::TW << LG) / 16), 0, s,
-                           a);
+        // four-step tiles: specialised two-stage kernel (scratch in the compute type), gz = grids
+        hipLaunchKernelGGL((tile_kernel<T, S, LG, MODE, CONJ>), dim3(gx, gy, gz), dim3((cfg<T, S>::TW << LG) / 16), 0,
+                           s, a);
     }
     else
     {
