@@ -116,7 +116,7 @@ def fft_r2c_1d(x: np.ndarray) -> np.ndarray:
     n = x.size
     out = np.empty(n + 2, dtype=np.float64)
     if lib().oracle_fft_r2c_1d(_ptr(x), _ptr(out), n) != 0:
-        raise ValueError("oracle_fft_r2c_1d: n must be a power of two >= 2")
+        raise ValueError("oracle_fft_r2c_1d: n must be even and >= 2")
     return out.view(np.complex128)
 
 

@@ -16,6 +16,9 @@
  *
  *   X[k] = sum_n x[n] exp(-2 pi i k n / N)                 (FFTW_FORWARD)
  *
+ * Lengths: any Nx >= 1 and any even Ny >= 2 (powers of two take a radix-2
+ * path, other lengths a mixed-radix one).
+ *
  * Pins (all exercised by tests/test_oracle.py):
  *   - the reference's only known-answer test, the 4x6 case with exact
  *     equality (test/src/test_shared_loop.cpp:15-34,53);
@@ -140,32 +143,87 @@ void oracle_twiddle(uint64_t m, uint64_t n, double *re, double *im)
 }
 
 /* ------------------------------------------------------------------------ */
-/* 1-D complex FFT, iterative radix-2 decimation in time with a precomputed
- * table of N/2 twiddles.  sign = -1 forward (FFTW_FORWARD), +1 backward.
- * Unnormalised.  n must be a power of two (n >= 1). */
+/* 1-D complex FFT, sign = -1 forward (FFTW_FORWARD), +1 backward,
+ * unnormalised, any length n >= 1.  Powers of two: iterative radix-2
+ * decimation in time with a table of n/2 twiddles.  Other lengths:
+ * recursive decimation in time by the smallest prime factor p of n,
+ *   X[k + q m] = sum_{r<p} w_n^{r (k + q m)} S_r[k],  n = p m,
+ * S_r the length-m transform of x[r + p j], with a table of all n twiddles
+ * (O(n sum of prime factors)). */
 typedef struct
 {
     size_t n;
-    double *tw; /* 2*(n/2) : w_n^m for m < n/2, forward sign */
+    int pow2;
+    double *tw; /* powers of two: w_n^m, m < n/2; otherwise w_n^m, m < n (forward sign) */
 } fft_tab;
 
 static int fft_tab_init(fft_tab *t, size_t n)
 {
     t->n = n;
     t->tw = NULL;
+    t->pow2 = (n & (n - 1)) == 0;
     if (n < 2) return 0;
-    t->tw = (double *)malloc(sizeof(double) * n);
+    const size_t cnt = t->pow2 ? n / 2 : n;
+    t->tw = (double *)malloc(sizeof(double) * 2 * cnt);
     if (!t->tw) return -1;
-    for (size_t m = 0; m < n / 2; ++m) oracle_twiddle(m, n, &t->tw[2 * m], &t->tw[2 * m + 1]);
+    for (size_t m = 0; m < cnt; ++m) oracle_twiddle(m, n, &t->tw[2 * m], &t->tw[2 * m + 1]);
     return 0;
 }
 
 static void fft_tab_free(fft_tab *t) { free(t->tw); t->tw = NULL; }
 
+static size_t smallest_factor(size_t n)
+{
+    if (n % 2 == 0) return 2;
+    for (size_t f = 3; f * f <= n; f += 2)
+        if (n % f == 0) return f;
+    return n;
+}
+
+/* length-n transform of in[0], in[s], ..., in[(n-1)s] into out[0..n);
+ * tw: the table of the full length N (n divides N); work: N complex */
+static void fft_rec(const double *tw, size_t N, const double *in, size_t s, double *out, size_t n, int sign,
+                    double *work)
+{
+    if (n == 1)
+    {
+        out[0] = in[0];
+        out[1] = in[1];
+        return;
+    }
+    const size_t p = smallest_factor(n), m = n / p, step = N / n;
+    for (size_t r = 0; r < p; ++r) fft_rec(tw, N, in + 2 * r * s, s * p, out + 2 * r * m, m, sign, work);
+    for (size_t q = 0; q < p; ++q)
+        for (size_t k = 0; k < m; ++k)
+        {
+            double ar = 0.0, ai = 0.0;
+            for (size_t r = 0; r < p; ++r)
+            {
+                const size_t e = (r * (k + q * m)) % n;
+                const double wr = tw[2 * e * step], wi = sign < 0 ? tw[2 * e * step + 1] : -tw[2 * e * step + 1];
+                const double xr = out[2 * (r * m + k)], xi = out[2 * (r * m + k) + 1];
+                ar += xr * wr - xi * wi;
+                ai += xr * wi + xi * wr;
+            }
+            work[2 * (k + q * m)] = ar;
+            work[2 * (k + q * m) + 1] = ai;
+        }
+    memcpy(out, work, sizeof(double) * 2 * n);
+}
+
 static void fft_c2c_tab(const fft_tab *t, double *a, int sign)
 {
     const size_t n = t->n;
     if (n < 2) return;
+    if (!t->pow2)
+    {
+        double *buf = (double *)malloc(sizeof(double) * 4 * n);
+        if (!buf) abort();
+        memcpy(buf, a, sizeof(double) * 2 * n);
+        fft_rec(t->tw, n, buf, 1, a, n, sign, buf + 2 * n);
+        free(buf);
+        return;
+    }
     /* bit reversal permutation */
     for (size_t i = 1, j = 0; i < n; ++i)
     {
@@ -253,7 +311,7 @@ static void fft_r2c_inplace(const fft_tab *tab, const double *wN, double *inout,
 
 int oracle_fft_r2c_1d(const double *in, double *out, size_t n)
 {
-    if (n < 2 || (n & (n - 1))) return -1;
+    if (n < 2 || n % 2) return -1;
     fft_tab t;
     if (fft_tab_init(&t, n / 2)) return -1;
     double *wN = (double *)malloc(sizeof(double) * n);
@@ -286,7 +344,7 @@ typedef struct
 
 void *oracle_loop_init(size_t nx, size_t ny, size_t ld)
 {
-    if (nx == 0 || ny < 2 || (nx & (nx - 1)) || (ny & (ny - 1)) || ld < 2 * (ny / 2 + 1)) return NULL;
+    if (nx == 0 || ny < 2 || ny % 2 || ld < 2 * (ny / 2 + 1)) return NULL;
     oracle_loop *p = (oracle_loop *)calloc(1, sizeof(oracle_loop));
     if (!p) return NULL;
     p->nx = nx;
@@ -461,7 +519,7 @@ void oracle_adjoint_r2c_2d_brute(const double *Y, size_t nx, size_t ny, double *
  * extension) of length Ny.  Y is not modified.  out pitch ld_out. */
 int oracle_adjoint_r2c_2d(const double *Y, size_t nx, size_t ny, double *out, size_t ld_out, int nthreads)
 {
-    if (nx == 0 || ny < 2 || (nx & (nx - 1)) || (ny & (ny - 1))) return -1;
+    if (nx == 0 || ny < 2 || ny % 2) return -1;
     if (nthreads > 0) omp_set_num_threads(nthreads);
     const size_t ncy = ny / 2 + 1;
     double *G = (double *)malloc(sizeof(double) * 2 * nx * ncy);

@@ -171,7 +171,28 @@ def test_adjoint_torch_fixtures_and_brute(nx, ny):
     assert np.max(np.abs(oracle.adjoint_brute(Y, ny) - ref)) < 1e-12 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("nx,ny", [(1, 2), (2, 2), (1, 8), (4, 8), (16, 4), (8, 16), (64, 128)])
+@pytest.mark.parametrize("n", [3, 5, 6, 7, 9, 12, 14, 15, 30, 49, 97, 210, 1155])
+def test_general_length_c2c_matches_numpy(n):
+    """Mixed-radix lengths (any n): forward and backward against numpy.fft."""
+    rng = np.random.default_rng(n)
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    ref = np.fft.fft(x)
+    assert np.max(np.abs(oracle.fft_c2c_1d(x) - ref)) <= 1e-14 * np.max(np.abs(ref))
+    assert np.max(np.abs(oracle.fft_c2c_1d(x, +1) - n * np.fft.ifft(x))) <= 1e-14 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("nx,ny", [(8, 14), (3, 10), (5, 6), (7, 30), (6, 4), (1, 2), (9, 18)])
+def test_general_size_r2c_brute_and_numpy(nx, ny):
+    """Non-power-of-two grids (the reference harness's default 8 x 14 among
+    them) against the long-double brute force and numpy.fft.rfft2."""
+    x = oracle.fill(nx, ny, seed=nx * 31 + ny, dist="normal")[:, :ny].copy()
+    got = oracle.r2c_2d(x)
+    assert rl2(got, oracle.dft_r2c_2d_brute(x)) < 1e-14
+    assert rl2(got, np.fft.rfft2(x)) < 1e-14
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 2), (2, 2), (1, 8), (4, 8), (16, 4), (8, 16), (64, 128), (8, 14), (5, 6),
+                                   (12, 30)])
 def test_c2r_matches_numpy_irfft2(nx, ny):
     """Inverse c2r of an arbitrary (non-Hermitian) half spectrum: Nx*Ny *
     numpy.fft.irfft2, which also drops the imaginary parts of the ky = 0 and
