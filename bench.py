@@ -64,7 +64,8 @@ def time_gpu(nx: int, ny: int, precision: str, steps: int, warmup: int, seed: in
     out = torch.empty((nx, ny // 2 + 1), dtype=torch.complex128 if precision == "f64" else torch.complex64,
                       device="cuda")
     stream = torch.cuda.current_stream()
-    plan = hpxfft_amd.Plan(nx, ny, ld_in=ld, precision=precision, stream=stream)
+    # effort "measure": the plan times the valid four-step splits of the column pass (outside the timed region)
+    plan = hpxfft_amd.Plan(nx, ny, ld_in=ld, precision=precision, effort="measure", stream=stream)
     for _ in range(warmup):
         plan.exec(x.data_ptr(), out.data_ptr())
     torch.cuda.synchronize()
@@ -89,9 +90,11 @@ def time_gpu(nx: int, ny: int, precision: str, steps: int, warmup: int, seed: in
             ph[k].append(plan.measurement(k))
     med = {k: sorted(v)[len(v) // 2] for k, v in ph.items()}
     desc = plan.describe()
+    plan_s = plan.measurement("plan")
     plan.close()
     return {"ms_per_step": ms, "wall_s": wall, "row_s": med["first_fftw"], "col_s": med["second_fftw"],
-            "col_a_s": med["column_a"], "col_b_s": med["column_b"], "out": out, "x": x, "plan": desc}
+            "col_a_s": med["column_a"], "col_b_s": med["column_b"], "out": out, "x": x, "plan": desc,
+            "plan_s": plan_s}
 
 
 def launches(nx: int, ny: int, s: int, r: dict) -> dict:
@@ -261,6 +264,27 @@ def bench_mixed(n: int = 1 << 15, reps: int = 3) -> dict:
     return out
 
 
+def bench_general(n: int = 3000, reps: int = 10) -> dict:
+    """A length that is not a power of two: 3000 x 3000 FP64 (radices 4, 2, 3, 5)
+    on the general mixed-radix kernels, 2-pass byte model as the headline."""
+    import torch
+    import hpxfft_amd
+
+    ld = 2 * (n // 2 + 1)
+    g = torch.Generator(device="cuda").manual_seed(6)
+    x = torch.empty((n, ld), dtype=torch.float64, device="cuda")
+    x[:, :n].uniform_(-1.0, 1.0, generator=g)
+    x[:, n:].zero_()
+    out = torch.empty((n, n // 2 + 1), dtype=torch.complex128, device="cuda")
+    p = hpxfft_amd.Plan(n, n, ld_in=ld, precision="f64")
+    ms = _events_ms(lambda: p.exec(x.data_ptr(), out.data_ptr()), reps)
+    desc = [ln.strip() for ln in p.describe().splitlines() if "general" in ln]
+    p.close()
+    B = alg_bytes(n, n, 8)["total"]
+    return {"config": f"{n}x{n} FP64", "ms": round(ms, 4), "GBps": round(B / (ms * 1e-3) / 1e9, 1),
+            "kernels": desc}
+
+
 def cpu_baseline(nx: int, ny: int, budget_s: float = 20.0):
     """The oracle's restatement of shared::loop::fft_2d_r2c_par on host cores."""
     import oracle
@@ -342,6 +366,8 @@ def main() -> int:
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                      "alg_bytes_per_launch": dom_bytes, "avg_launch_s": dom_s},
         "launches_s": {k: v[1] for k, v in per_launch.items()},
+        "plan": {"effort": "measure", "seconds": round(r64["plan_s"], 4),
+                 "column_pass": [ln.strip() for ln in r64["plan"].splitlines() if "four-step" in ln or "autotuned" in ln]},
         "phases_s": {"row_r2c": r64["row_s"], "column_c2c": r64["col_s"]},
     }
     result["fp64_vector_frac"] = round(result["gflops"] / (FP64_VEC_TFLOPS * 1e3), 4)
@@ -380,6 +406,9 @@ def main() -> int:
         torch.cuda.empty_cache()
         result["mixed_2e15"] = bench_mixed()
         progress("2^15 mixed precision done")
+        torch.cuda.empty_cache()
+        result["general_length"] = bench_general()
+        progress("general length done")
         torch.cuda.empty_cache()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(nx, ny)

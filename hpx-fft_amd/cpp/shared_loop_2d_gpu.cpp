@@ -7,8 +7,10 @@
 // same console report (:69-93), same ';'-separated CSV appended to
 // runtimes/runtimes_hpx_shared_loop.txt (:95-111), same plan file
 // plans/plan_hpx_shared_loop.txt (:114-128).  The n_threads column holds the
-// string "gpu".  Sizes must be powers of two (otherwise the class throws
-// std::invalid_argument and the program exits with status 2).
+// string "gpu".  Sizes: any Nx and even Ny whose lengths the library
+// supports (powers of two, or prime factors <= 61 within one workgroup's
+// LDS); otherwise the class throws std::invalid_argument and the program
+// exits with status 2.
 #include "hpxfft/shared/loop.hpp"
 
 #include <chrono>

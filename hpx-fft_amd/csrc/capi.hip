@@ -7,6 +7,7 @@
 // trans_values_vec_, which the fused column pass does not need at full
 // size), twiddle tables, timing events and the measurement map.
 #include "launch.hpp"
+#include "fft_general.hpp"
 #include "batched.hpp"
 #include "../../include/hpxfft_hip.h"
 
@@ -186,6 +187,12 @@ struct hpxfft_plan_s
     int lgP = 0, lgQ = 0;
     int W = 8;
     size_t CW = 0;   // chunk width (columns) of the four-step scratch
+    // lengths that are not powers of two: general kernels (F64 / F32 forward)
+    bool gen_row = false, gen_col = false;
+    hpxfft_dev::GenPlan grow{}, gcol{};
+    int gen_w = 1;                 // general column kernel: adjacent columns per workgroup
+    double *gen_tab = nullptr;     // exact FP64 tables w_n^m: rows L = Ny/2, split Ny, columns Nx
+    size_t gtab_L = 0, gtab_2 = 0, gtab_N = 0;   // their offsets (complex entries)
     // device memory
     dev_tables tabs;
     void *scratch = nullptr;
@@ -201,7 +208,7 @@ struct hpxfft_plan_s
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // start, rows done, columns done, column phase A done
     bool ran = false;
-    bool col_split = false;      // ev[3] valid: last run was a single-grid four-step forward
+    bool col_split = false;      // ev[3] valid: last run was a four-step forward
     double h2d = 0.0, d2h = 0.0;
     double plan_seconds = 0.0, plan_flops = 0.0;
     std::map<std::string, double> extra;
@@ -215,6 +222,7 @@ struct hpxfft_plan_s
     int bntasks[2][hpxfft_host::BATCH_NCFG] = {};
     int bblocks[2][hpxfft_host::BATCH_NCFG] = {};
     double btune_ms[2][hpxfft_host::BATCH_NCFG] = {};
+    double split_ms[COL_TILE_MAX_LG + 1] = {};   // four-step: column-pass ms per tried log2 P (effort >= measure)
     unsigned *bticket = nullptr;                            // two work-queue counters
     void *bstage = nullptr;                                 // pinned staging of the grid descriptors
 };
@@ -227,6 +235,43 @@ const void *tw_ptr(const hpxfft_plan_s *p, size_t N)
     return p->tabs.tw32 + 2 * N;
 }
 const void *tw64_ptr(const hpxfft_plan_s *p, size_t N) { return p->tabs.tw64 + 2 * N; }
+
+// Stockham radices for a length that is not a power of two: 4s, then a 2,
+// then odd primes ascending; false if a prime factor exceeds GEN_MAX_RADIX
+bool gen_factor(size_t n, hpxfft_dev::GenPlan &g)
+{
+    g = hpxfft_dev::GenPlan{};
+    g.n = (int)n;
+    size_t m = n;
+    auto push = [&](int r) {
+        if (g.nstages >= hpxfft_dev::GEN_MAX_STAGES) return false;
+        g.radix[g.nstages++] = r;
+        return true;
+    };
+    while (m % 4 == 0) { if (!push(4)) return false; m /= 4; }
+    if (m % 2 == 0) { if (!push(2)) return false; m /= 2; }
+    for (size_t f = 3; m > 1; f += 2)
+    {
+        while (m % f == 0)
+        {
+            if (f > (size_t)hpxfft_dev::GEN_MAX_RADIX || !push((int)f)) return false;
+            m /= f;
+        }
+        if (f * f > m && m > 1)
+        {
+            if (m > (size_t)hpxfft_dev::GEN_MAX_RADIX || !push((int)m)) return false;
+            m = 1;
+        }
+    }
+    return true;
+}
+
+std::string radices(const hpxfft_dev::GenPlan &g)
+{
+    std::string r;
+    for (int i = 0; i < g.nstages; ++i) r += (i ? "x" : "") + std::to_string(g.radix[i]);
+    return r.empty() ? "none" : r;
+}
 
 std::string describe(const hpxfft_plan_s *p)
 {
@@ -252,13 +297,22 @@ std::string describe(const hpxfft_plan_s *p)
         os << "  total real points " << pts << "\n";
         return os.str();
     }
-    const geom rg = row_geom(p->prec, p->lgL);
     os << "hpxfft-hip plan (gfx950): Nx=" << p->nx << " Ny=" << p->ny << " batch=" << p->batch << " precision "
        << prec_name(p->prec) << " effort " << p->effort << "\n";
-    os << "  row r2c kernel: " << (size_t(1) << p->lgL) << "-point complex Stockham per row + r2c split, "
-       << rg.nseq << " row(s)/workgroup, " << rg.nt << " threads, " << rg.E << " elements/thread, radix "
-       << (rg.E < 16 ? rg.E : 16) << (rg.split ? ", LDS exchange in re/im halves" : "") << "\n";
-    if (!p->four_step)
+    if (p->gen_row)
+        os << "  row r2c kernel (general length): " << p->ny / 2 << "-point mixed-radix Stockham in LDS, radices "
+           << radices(p->grow) << ", 1 row/workgroup, " << hpxfft_dev::GEN_NT << " threads + r2c split\n";
+    else
+    {
+        const geom rg = row_geom(p->prec, p->lgL);
+        os << "  row r2c kernel: " << (size_t(1) << p->lgL) << "-point complex Stockham per row + r2c split, "
+           << rg.nseq << " row(s)/workgroup, " << rg.nt << " threads, " << rg.E << " elements/thread, radix "
+           << (rg.E < 16 ? rg.E : 16) << (rg.split ? ", LDS exchange in re/im halves" : "") << "\n";
+    }
+    if (p->gen_col)
+        os << "  column c2c kernel (general length): " << p->nx << "-point mixed-radix Stockham in LDS, radices "
+           << radices(p->gcol) << ", " << p->gen_w << " adjacent columns/workgroup\n";
+    else if (!p->four_step)
     {
         const geom cg = col_geom(p->prec, p->lgx);
         os << "  column c2c kernel: " << p->nx << "-point, one tile of " << cg.nseq << " adjacent columns x all rows, "
@@ -274,6 +328,15 @@ std::string describe(const hpxfft_plan_s *p)
            << ga.nseq << " columns, " << ga.nt << " threads\n";
         os << "    phase B: " << (1 << p->lgP) << "-point DFTs, to rows Q*k1+k2, tiles of " << gb.nseq << " columns, "
            << gb.nt << " threads\n";
+        bool tuned = false;
+        for (int l = 0; l <= COL_TILE_MAX_LG; ++l) tuned = tuned || p->split_ms[l] > 0.0;
+        if (tuned)
+        {
+            os << "    split autotuned (column pass ms):";
+            for (int l = 0; l <= COL_TILE_MAX_LG; ++l)
+                if (p->split_ms[l] > 0.0) os << " P=" << (1 << l) << ": " << p->split_ms[l];
+            os << "\n";
+        }
     }
     const double sr = (double)storage_bytes(p->prec);
     const double H = (double)p->nx * (double)p->ncols * 2.0 * sr;
@@ -281,6 +344,13 @@ std::string describe(const hpxfft_plan_s *p)
     os << "  algorithmic bytes per grid (2-pass model): " << (long long)B << "\n";
     os << "  plan_flops (2.5 Ny log2 Ny per row + 5 Nx log2 Nx per column): " << p->plan_flops << "\n";
     return os.str();
+}
+
+double elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
+    return (double)ms * 1e-3;
 }
 
 int alloc_events(hpxfft_plan_s *p)
@@ -305,6 +375,21 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
     a.out_exp = col_exp;
     a.rexp_dist = (long long)p->nx;
     a.cexp_dist = (long long)p->ncols;
+    if (p->gen_col)
+    {
+        // general length (forward, in place on spec_out = spec_in)
+        a.out = spec_out;
+        const dim3 grid((unsigned)((p->ncols + p->gen_w - 1) / p->gen_w), (unsigned)p->batch);
+        const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab) + p->gtab_N;
+        if (p->prec == P_F64)
+            hipLaunchKernelGGL(hpxfft_dev::gen_col_kernel<double>, grid, dim3(hpxfft_dev::GEN_NT), 0, p->stream, a,
+                               p->gcol, tab, p->gen_w);
+        else
+            hipLaunchKernelGGL(hpxfft_dev::gen_col_kernel<float>, grid, dim3(hpxfft_dev::GEN_NT), 0, p->stream, a,
+                               p->gcol, tab, p->gen_w);
+        HIPCHK(hipGetLastError());
+        return HPXFFT_OK;
+    }
     if (!p->four_step)
     {
         a.in = spec_in;
@@ -376,9 +461,24 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     r.in_exp = s16 ? in_exp : nullptr;
     r.out_exp = s16 ? p->mid_exp : nullptr;
     r.exp_dist = (long long)p->nx;
-    const geom rg = row_geom(p->prec, p->lgL);
-    const unsigned gx = (unsigned)((p->nx + rg.nseq - 1) / rg.nseq);
-    HIPCHK(row_r2c(p->prec, p->lgL, r, gx, (unsigned)p->batch, p->stream));
+    if (p->gen_row)
+    {
+        const dim3 grid((unsigned)p->nx, (unsigned)p->batch);
+        const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
+        if (p->prec == P_F64)
+            hipLaunchKernelGGL(hpxfft_dev::gen_row_r2c_kernel<double>, grid, dim3(hpxfft_dev::GEN_NT), 0, p->stream,
+                               r, p->grow, tab + p->gtab_L, tab + p->gtab_2);
+        else
+            hipLaunchKernelGGL(hpxfft_dev::gen_row_r2c_kernel<float>, grid, dim3(hpxfft_dev::GEN_NT), 0, p->stream,
+                               r, p->grow, tab + p->gtab_L, tab + p->gtab_2);
+        HIPCHK(hipGetLastError());
+    }
+    else
+    {
+        const geom rg = row_geom(p->prec, p->lgL);
+        const unsigned gx = (unsigned)((p->nx + rg.nseq - 1) / rg.nseq);
+        HIPCHK(row_r2c(p->prec, p->lgL, r, gx, (unsigned)p->batch, p->stream));
+    }
     HIPCHK(hipEventRecord(p->ev[1], p->stream));
     if (int rc = run_columns(p, d_out, d_out, false, s16 ? col_exp : nullptr, p->ev[3])) return rc;
     HIPCHK(hipEventRecord(p->ev[2], p->stream));
@@ -388,13 +488,97 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     return HPXFFT_OK;
 }
 
+// four-step splits Nx = P*Q with both sub-lengths in [2^COL_TILE_MIN_LG, 2^COL_TILE_MAX_LG]
+void split_range(int lgx, int &lo, int &hi)
+{
+    lo = std::max(COL_TILE_MIN_LG, lgx - COL_TILE_MAX_LG);
+    hi = std::min(COL_TILE_MAX_LG, lgx - COL_TILE_MIN_LG);
+}
+
+// effort >= measure: time the column pass of every valid split on a zeroed
+// spectrum of the plan's shape (forward, in place) and keep the fastest.
+// If the temporary spectrum cannot be allocated the default split stays.
+int tune_split(hpxfft_plan_s *p)
+{
+    int lo, hi;
+    split_range(p->lgx, lo, hi);
+    if (lo >= hi) return HPXFFT_OK;
+    const size_t bytes = p->batch * p->dist_out * 2 * storage_bytes(p->prec);
+    void *tmp = nullptr;
+    if (hipMalloc(&tmp, bytes) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        return HPXFFT_OK;
+    }
+    const bool s16 = p->prec == P_F16 || p->prec == P_BF16;
+    int rc = HPXFFT_OK;
+    int best = p->lgP;
+    double best_ms = 0.0;
+    hipError_t e = hipMemsetAsync(tmp, 0, bytes, p->stream);
+    if (e == hipSuccess && s16) e = hipMemsetAsync(p->mid_exp, 0, sizeof(int) * p->nx * p->batch, p->stream);
+    for (int l = lo; l <= hi && e == hipSuccess && rc == HPXFFT_OK; ++l)
+    {
+        p->lgP = l;
+        p->lgQ = p->lgx - l;
+        constexpr int REPS = 3;
+        rc = run_columns(p, tmp, tmp, false, s16 ? p->col_exp_own : nullptr);   // warm-up
+        if (rc == HPXFFT_OK) e = hipEventRecord(p->ev[0], p->stream);
+        for (int r = 0; r < REPS && rc == HPXFFT_OK && e == hipSuccess; ++r)
+            rc = run_columns(p, tmp, tmp, false, s16 ? p->col_exp_own : nullptr);
+        if (rc == HPXFFT_OK && e == hipSuccess) e = hipEventRecord(p->ev[2], p->stream);
+        if (rc == HPXFFT_OK && e == hipSuccess) e = hipEventSynchronize(p->ev[2]);
+        if (rc == HPXFFT_OK && e == hipSuccess)
+        {
+            const double ms = elapsed(p->ev[0], p->ev[2]) * 1e3 / REPS;
+            p->split_ms[l] = ms;
+            if (best_ms == 0.0 || ms < best_ms) { best_ms = ms; best = l; }
+        }
+    }
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipFree(tmp);
+    p->lgP = best;
+    p->lgQ = p->lgx - best;
+    if (rc != HPXFFT_OK) return rc;
+    HIPCHK(e);
+    return HPXFFT_OK;
+}
+
 int plan_common(hpxfft_plan_s *p)
 {
-    if (!is_pow2(p->nx) || !is_pow2(p->ny) || p->ny < 2)
-        return fail(HPXFFT_UNSUPPORTED_SIZE, "Nx and Ny must be powers of two with Ny >= 2 (got " +
-                                                 std::to_string(p->nx) + " x " + std::to_string(p->ny) + ")");
-    p->lgx = ilog2(p->nx);
-    p->lgL = ilog2(p->ny / 2);
+    const std::string dims = std::to_string(p->nx) + " x " + std::to_string(p->ny);
+    if (p->nx == 0 || p->ny < 2 || p->ny % 2)
+        return fail(HPXFFT_UNSUPPORTED_SIZE, "need Nx >= 1 and an even Ny >= 2 (got " + dims + ")");
+    p->gen_row = !is_pow2(p->ny);
+    p->gen_col = !is_pow2(p->nx);
+    if (p->gen_row || p->gen_col)
+    {
+        // lengths that are not powers of two: general kernels, FP64 / FP32
+        if (p->prec != P_F64 && p->prec != P_F32)
+            return fail(HPXFFT_UNSUPPORTED_SIZE, "lengths other than powers of two need an F64 or F32 plan (got " +
+                                                     dims + ")");
+        const size_t cap = p->prec == P_F64 ? hpxfft_dev::gen_max_points<double>()
+                                            : hpxfft_dev::gen_max_points<float>();
+        if (p->gen_row && (p->ny / 2 > cap || !gen_factor(p->ny / 2, p->grow)))
+            return fail(HPXFFT_UNSUPPORTED_SIZE, "row length Ny/2 = " + std::to_string(p->ny / 2) +
+                                                     " needs prime factors <= " +
+                                                     std::to_string(hpxfft_dev::GEN_MAX_RADIX) + " and at most " +
+                                                     std::to_string(cap) + " points (got " + dims + ")");
+        if (p->gen_col && (p->nx > cap || !gen_factor(p->nx, p->gcol)))
+            return fail(HPXFFT_UNSUPPORTED_SIZE, "column length Nx = " + std::to_string(p->nx) +
+                                                     " needs prime factors <= " +
+                                                     std::to_string(hpxfft_dev::GEN_MAX_RADIX) + " and at most " +
+                                                     std::to_string(cap) + " points (got " + dims + ")");
+        if (p->gen_col)
+        {
+            // one 128-byte row segment per workgroup where the LDS allows
+            const size_t wmax = 128 / (2 * storage_bytes(p->prec));
+            size_t w = 1;
+            while (w * 2 <= wmax && w * 2 * p->nx <= cap) w *= 2;
+            p->gen_w = (int)w;
+        }
+    }
+    p->lgx = p->gen_col ? 0 : ilog2(p->nx);
+    p->lgL = p->gen_row ? 0 : ilog2(p->ny / 2);
     if (p->lgL > ROW_MAX_LG || p->lgx > 2 * COL_TILE_MAX_LG)
         return fail(HPXFFT_UNSUPPORTED_SIZE, "grid too large: Nx <= 65536 and Ny <= 32768 are supported");
     p->ncols = p->ny / 2 + 1;
@@ -407,9 +591,32 @@ int plan_common(hpxfft_plan_s *p)
     if (p->dist_in % 2 != 0) return fail(HPXFFT_INVALID_ARG, "dist_in must be even");
     if (int rc = check_device(p->device)) return rc;
     if (int rc = get_tables(p->device, p->tabs)) return rc;
+    if (p->gen_row || p->gen_col)
+    {
+        // exact FP64 tables (the long-double evaluation of the power-of-two tables)
+        const size_t L = p->gen_row ? p->ny / 2 : 0, N2 = p->gen_row ? p->ny : 0, NX = p->gen_col ? p->nx : 0;
+        p->gtab_L = 0;
+        p->gtab_2 = L;
+        p->gtab_N = L + N2;
+        std::vector<double> h(2 * (L + N2 + NX));
+        auto fill = [&](size_t off, size_t n) {
+            for (size_t m = 0; m < n; ++m)
+            {
+                long double c, sn;
+                twiddle_ld(m, n, c, sn);
+                h[2 * (off + m)] = (double)c;
+                h[2 * (off + m) + 1] = -(double)sn;
+            }
+        };
+        fill(p->gtab_L, L);
+        fill(p->gtab_2, N2);
+        fill(p->gtab_N, NX);
+        HIPCHK(hipMalloc(&p->gen_tab, sizeof(double) * h.size()));
+        HIPCHK(hipMemcpy(p->gen_tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+    }
     p->W = col_geom(p->prec, p->lgx).nseq;
     const int dmax = col_direct_max_lg(p->prec);
-    p->four_step = p->lgx > dmax;
+    p->four_step = !p->gen_col && p->lgx > dmax;
     if (p->four_step)
     {
         p->lgP = p->lgx / 2;
@@ -430,13 +637,17 @@ int plan_common(hpxfft_plan_s *p)
         HIPCHK(hipMalloc(&p->col_exp_own, sizeof(int) * p->ncols * p->batch));
         if (p->four_step)
         {
-            HIPCHK(hipMalloc(&p->colpart, sizeof(unsigned) * p->ncols * ((size_t)1 << p->lgP) * p->batch));
+            int lo, hi;   // room for the largest P any split may use
+            split_range(p->lgx, lo, hi);
+            HIPCHK(hipMalloc(&p->colpart, sizeof(unsigned) * p->ncols * ((size_t)1 << std::max(hi, p->lgP)) * p->batch));
             HIPCHK(hipMalloc(&p->colmax, sizeof(unsigned) * p->ncols * p->batch));
         }
     }
     (void)sb;
     if (int rc = alloc_events(p)) return rc;
-    const double lx = (double)p->lgx, ly = (double)ilog2(p->ny);
+    if (p->four_step && p->effort >= HPXFFT_MEASURE)
+        if (int rc = tune_split(p)) return rc;
+    const double lx = std::log2((double)p->nx), ly = std::log2((double)p->ny);
     p->plan_flops = (double)p->batch * ((double)p->nx * 2.5 * (double)p->ny * ly +
                                         (double)p->ncols * 5.0 * (double)p->nx * lx);
     return HPXFFT_OK;
@@ -455,6 +666,7 @@ void free_plan(hpxfft_plan_s *p)
     (void)hipFree(p->mid_exp);
     (void)hipFree(p->colpart);
     (void)hipFree(p->colmax);
+    (void)hipFree(p->gen_tab);
     (void)hipFree(p->col_exp_own);
     (void)hipFree(p->adj_work);
     (void)hipFree(p->host_buf);
@@ -464,13 +676,6 @@ void free_plan(hpxfft_plan_s *p)
     (void)hipFree(p->bticket);
     if (p->bstage) (void)hipHostFree(p->bstage);
     delete p;
-}
-
-double elapsed(hipEvent_t a, hipEvent_t b)
-{
-    float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return 0.0;
-    return (double)ms * 1e-3;
 }
 
 }   // namespace
@@ -620,6 +825,8 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     if (!d_spec || !d_real) return fail(HPXFFT_INVALID_ARG, "null device pointer");
     if (plan->prec != P_F64 && plan->prec != P_F32)
         return fail(HPXFFT_INVALID_ARG, std::string(what) + " is implemented for F64 and F32 plans");
+    if (plan->gen_row || plan->gen_col)
+        return fail(HPXFFT_UNSUPPORTED_SIZE, std::string(what) + " is implemented for power-of-two Nx and Ny");
     device_guard g(plan->device);
     const size_t sb = storage_bytes(plan->prec);
     const size_t bytes = plan->batch * plan->dist_out * 2 * sb;

@@ -70,6 +70,8 @@ struct RowArgs
     long long exp_dist;    // between grids, for in_exp/out_exp
 };
 
+// column-pass arguments; MODE 0 = direct column kernel, 1 = four-step phase A
+// (spectrum -> scratch), 2 = four-step phase B (scratch -> spectrum)
 struct ColArgs
 {
     const void *in;    // spectrum (MODE 0,1) or scratch (MODE 2)

@@ -37,7 +37,9 @@ typedef enum
 {
     HPXFFT_OK = 0,
     HPXFFT_INVALID_ARG = 1,      /* bad pointer, flag, layout, or call order */
-    HPXFFT_UNSUPPORTED_SIZE = 2, /* Nx or Ny not a power of two, or out of range */
+    HPXFFT_UNSUPPORTED_SIZE = 2, /* size outside what the kernels cover: powers of two (Nx <= 2^16,
+                                    Ny <= 2^15), or for F64/F32 forward transforms any Nx and even Ny
+                                    with prime factors <= 61 within 4096 (F64) / 8192 (F32) points */
     HPXFFT_NO_DEVICE = 3,        /* no HIP device visible */
     HPXFFT_OUT_OF_MEMORY = 4,
     HPXFFT_HIP_ERROR = 5,
@@ -123,9 +125,10 @@ int hpxfft_exec_c2r_2d(hpxfft_plan plan, const void *d_spec, void *d_real);
 /* Replaces hpxfft::shared::loop::get_measurement (loop.cpp:192): seconds of
  * the last exec for "total", "first_fftw", "first_trans", "second_fftw",
  * "second_trans" (the transposes are fused into the column pass and read
- * 0), "h2d", "d2h"; "plan" (planning seconds) and "plan_flops"; and, for a
- * single grid whose column pass runs as two launches, "column_a" and
- * "column_b" (seconds of each launch; 0 otherwise).  Unknown keys
+ * 0), "h2d", "d2h"; "plan" (planning seconds) and "plan_flops"; and, when
+ * the column pass runs as two launches (Nx > 1024, or > 512 for 16-bit
+ * storage), "column_a" and "column_b" (seconds of each launch of the last
+ * forward exec; 0 otherwise).  Unknown keys
  * give 0.0, as std::map::operator[] does in the reference.  Waits for the
  * last exec to finish. */
 int hpxfft_get_measurement(hpxfft_plan plan, const char *key, double *value);

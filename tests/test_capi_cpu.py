@@ -71,10 +71,29 @@ def plan_status(nx, ny, batch=1, ld_in=None, dist_in=None, ld_out=None, dist_out
     return st
 
 
-@pytest.mark.parametrize("nx,ny", [(3, 16), (16, 14), (16, 1), (0, 16), (16, 0), (1 << 17, 4), (4, 1 << 16)])
+@pytest.mark.parametrize("nx,ny", [(16, 1), (16, 15), (0, 16), (16, 0), (1 << 17, 4), (4, 1 << 16),
+                                   (16, 134), (67, 16), (16, 2 * 8193), (6000, 8)])
 def test_unsupported_sizes(nx, ny):
+    """Odd Ny, zero sizes, powers of two beyond the kernels' range, and other
+    lengths with a prime factor above 61 or beyond one workgroup's LDS."""
     assert plan_status(nx, ny) == _lib.UNSUPPORTED_SIZE
     assert hpxfft_amd.last_error()
+
+
+@pytest.mark.parametrize("prec", [2, 3])
+def test_general_lengths_need_f64_or_f32(prec):
+    assert plan_status(8, 14, prec=prec) == _lib.UNSUPPORTED_SIZE
+    assert "F64 or F32" in hpxfft_amd.last_error()
+
+
+@pytest.mark.parametrize("nx,ny", [(8, 14), (3, 16), (1000, 64), (64, 1000), (4095, 2)])
+def test_general_lengths_pass_validation(nx, ny):
+    """Lengths that are not powers of two are accepted (without a device the
+    plan then stops at the device check)."""
+    st = plan_status(nx, ny)
+    if st == _lib.OK:
+        pytest.skip("a HIP device is present")
+    assert st == _lib.NO_DEVICE, hpxfft_amd.last_error()
 
 
 def test_bad_arguments():
@@ -138,8 +157,8 @@ def test_cpp_harness_argument_errors(tmp_path):
     r = subprocess.run([exe, "--plan=bogus", "--nx", "4", "--ny", "4"], capture_output=True, text=True, timeout=60,
                        cwd=tmp_path)
     assert r.returncode == 2 and "Invalid FFTW plan flag string" in r.stderr
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=60, cwd=tmp_path)   # default 8 x 14
-    assert r.returncode == 2 and "powers of two" in r.stderr
+    r = subprocess.run([exe, "--nx", "8", "--ny", "134"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert r.returncode == 2 and "prime factors" in r.stderr       # Ny/2 = 67
 
 
 def test_cpp_known_answer_without_device():

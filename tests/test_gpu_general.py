@@ -1,0 +1,89 @@
+"""Lengths that are not powers of two (the general mixed-radix kernels) on the GPU.
+
+The reference accepts any even Ny and any Nx through FFTW; the general
+kernels cover lengths with prime factors <= 61 that fit one workgroup's LDS.
+Checks against the oracle (itself pinned to brute force and numpy for these
+lengths): FP64 <= 1e-12, FP32 <= 1e-5 relative L2, with general rows and/or
+general columns combined with every power-of-two kernel family; batches;
+the reference harness at its default 8 x 14; and that the adjoint reports
+these sizes as unsupported.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import REPO
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(REPO, "hpx-fft_amd", "bin")
+
+SIZES = [
+    (8, 14),        # the reference harness default
+    (3, 10), (5, 6), (7, 30), (1, 6), (4095, 2), (12, 100), (96, 96),
+    (1000, 64),     # general columns, power-of-two rows
+    (64, 1000),     # general rows, direct power-of-two columns
+    (3000, 8), (6, 8190),
+    (4096, 30),     # general rows, four-step columns
+    (10, 4096),     # general columns, long power-of-two rows
+]
+
+
+def _run(x_pad, nx, ny, prec, batch=1):
+    import torch
+    import hpxfft_amd
+    dt = torch.float64 if prec == "f64" else torch.float32
+    d = torch.from_numpy(x_pad).to(device="cuda", dtype=dt).contiguous()
+    p = hpxfft_amd.Plan(nx, ny, batch=batch, precision=prec)
+    p.exec(d.data_ptr(), d.data_ptr())
+    torch.cuda.synchronize()
+    p.close()
+    return d.cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("nx,ny", SIZES)
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5)])
+def test_general_parity(gpu, nx, ny, prec, tol):
+    print(f"general {prec} {nx}x{ny}", flush=True)
+    x = oracle.fill(nx, ny, seed=5000 + nx + ny, dist="upm1")
+    ref = oracle.r2c_2d(x, ny)
+    got = _run(x, nx, ny, prec)[:, : 2 * (ny // 2 + 1)].copy().view(np.complex128)
+    err = oracle.rel_l2(got, ref)
+    print(f"  rel-L2 = {err:.3e}", flush=True)
+    assert err <= tol, err
+
+
+def test_general_fp32_beyond_fp64_lds(gpu):
+    """FP32 holds twice the points per workgroup: Nx = 6000 (FP64 rejects it)."""
+    nx, ny = 6000, 8
+    x = oracle.fill(nx, ny, seed=11, dist="normal")
+    got = _run(x, nx, ny, "f32")[:, : 2 * (ny // 2 + 1)].copy().view(np.complex128)
+    assert oracle.rel_l2(got, oracle.r2c_2d(x, ny)) <= 1e-5
+
+
+def test_general_batched(gpu):
+    nx, ny, batch = 30, 42, 3
+    ld = 2 * (ny // 2 + 1)
+    xs = [oracle.fill(nx, ny, seed=300 + b, dist="upm1") for b in range(batch)]
+    got = _run(np.concatenate(xs), nx, ny, "f64", batch=batch)
+    for b, x in enumerate(xs):
+        g = got[b * nx:(b + 1) * nx, :ld].copy().view(np.complex128)
+        assert oracle.rel_l2(g, oracle.r2c_2d(x, ny)) <= 1e-12, b
+
+
+def test_adjoint_reports_general_lengths_unsupported(gpu):
+    import torch
+    import hpxfft_amd
+    Y = torch.zeros((8, 8), dtype=torch.complex128, device="cuda")
+    with pytest.raises(hpxfft_amd.HpxfftError, match="power-of-two"):
+        hpxfft_amd.rfft2_adjoint(Y, 14)
+
+
+def test_cpp_harness_default_size(gpu, tmp_path):
+    """No arguments: the reference's default 8 x 14 ramp, X[0][0] = Nx Ny (Ny-1)/2 = 728."""
+    r = subprocess.run([os.path.join(BIN, "shared_loop_2d_gpu"), "--result", "1"], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "(728 0)" in r.stdout, r.stdout

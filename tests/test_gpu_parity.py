@@ -97,6 +97,27 @@ def test_parity_batched_strided(gpu, nx, ny, batch, prec, tol):
         assert np.all(out[b * dist_out + nx * ld_out: (b + 1) * dist_out] == complex(7.0, -7.0)), b
 
 
+@pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
+@pytest.mark.parametrize("nx,ny", [(4096, 256), (2048, 64)])
+def test_measure_effort_autotunes_split(gpu, nx, ny, prec, tol):
+    """effort "measure" times every valid four-step split Nx = P*Q of the
+    column pass and keeps the fastest: the plan says so, and the result is
+    the oracle's whatever the choice."""
+    import torch
+    import hpxfft_amd
+    x = oracle.fill(nx, ny, seed=77, dist="upm1")
+    dt = torch.float64 if prec == "f64" else torch.float32
+    d = torch.from_numpy(x).to(device="cuda", dtype=dt)
+    p = hpxfft_amd.Plan(nx, ny, precision=prec, effort="measure")
+    desc = p.describe()
+    assert "split autotuned" in desc, desc
+    p.exec(d.data_ptr(), d.data_ptr())
+    torch.cuda.synchronize()
+    got = d.cpu().numpy().astype(np.float64)[:, : 2 * (ny // 2 + 1)].copy().view(np.complex128)
+    p.close()
+    assert oracle.rel_l2(got, oracle.r2c_2d(x, ny)) <= tol
+
+
 @pytest.mark.parametrize("nx,ny", SIZES)
 def test_parity_f32(gpu, nx, ny):
     print(f"f32 {nx}x{ny}", flush=True)
