@@ -129,7 +129,7 @@ static int run_all()
         hpxfft::shared::loop f;
         try
         {
-            f.initialize(hpxfft::shared::vector_2d(8, 16, 1.0), "estimate");   // Ny = 14
+            f.initialize(hpxfft::shared::vector_2d(8, 136, 1.0), "estimate");   // Ny = 134: Ny/2 = 67 is prime > 61
         }
         catch (const std::invalid_argument &)
         {
