@@ -230,6 +230,45 @@ __device__ __forceinline__ void r2c_pair(SO *orow, int L, int k, cpx<T> A, cpx<T
     stor<SO>::st(orow + 2 * (L - k), x1);
 }
 
+// z * (-i)^k, exact
+template <typename T> __device__ __forceinline__ cpx<T> rot_mi(cpx<T> z, int k)
+{
+    switch (k & 3)
+    {
+    case 0: return z;
+    case 1: return cpx<T>{z.y, -z.x};
+    case 2: return cpx<T>{-z.x, -z.y};
+    default: return cpx<T>{-z.y, z.x};
+    }
+}
+
+// final-stage twiddles of butterflies j and jp = NSL - j: w_{2L}^{2 r j} and
+// w_{2L}^{2 r jp} = (-i)^{4 r / RL} conj(w_{2L}^{2 r j}); for RL <= 4 the
+// second is formed exactly from the first (the table is octant-symmetric, so
+// the values are the table's own), saving a load per factor
+template <typename T, int L, int RL>
+__device__ __forceinline__ void fused_twiddles(cpx<T> *a, cpx<T> *c, int j, const cpx<T> *__restrict__ tw)
+{
+    constexpr int NSL = L / RL;
+    const int jp = NSL - j;
+#pragma unroll
+    for (int r = 1; r < RL; ++r)
+    {
+        const cpx<T> wa = tw[2 * r * j];
+        a[r] = cmul(a[r], wa);
+        if constexpr (RL <= 4) c[r] = cmul(c[r], rot_mi(conjc(wa), 4 * r / RL));
+        else c[r] = cmul(c[r], tw[(2 * r * jp) & (2 * L - 1)]);
+    }
+}
+
+// r2c twiddle w_{2L}^{j + q NSL}: for RL = 2, w^{j + NSL} = -i w^j exactly
+template <typename T, int L, int RL>
+__device__ __forceinline__ cpx<T> split_twiddle(const cpx<T> &wj, int j, int q, const cpx<T> *__restrict__ tw)
+{
+    if constexpr (RL == 2) return q == 0 ? wj : rot_mi(wj, 1);
+    else return tw[j + q * (L / RL)];
+}
+
 template <typename T>
 __device__ __forceinline__ float cmax(cpx<T> a, cpx<T> b)
 {
@@ -279,15 +318,9 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
 #pragma unroll
         for (int i = 0; i < B::NMAIN; ++i)
         {
-            const int j = t + i * NT;
-            const int jp = NSL - j;   // == NSL for j == 0 (then c == a)
+            const int j = t + i * NT;   // its mirror NSL - j is NSL for j == 0 (then c == a)
             // final-stage twiddles w_L^{r b} = w_{2L}^{2 r b}
-#pragma unroll
-            for (int r = 1; r < RL; ++r)
-            {
-                a[i][r] = cmul(a[i][r], tw[2 * r * j]);
-                c[i][r] = cmul(c[i][r], tw[(2 * r * jp) & (2 * L - 1)]);
-            }
+            fused_twiddles<T, L, RL>(a[i], c[i], j, tw);
             dftR<T, RL>(a[i]);
             dftR<T, RL>(c[i]);
             // a[q] = Z[j + q NSL], c[q] = Z[jp + q NSL]
@@ -302,8 +335,15 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
             }
             else
             {
+                const cpx<T> wj = tw[j];
 #pragma unroll
-                for (int q = 0; q < RL; ++q) r2c_pair<T, SO>(orow, L, j + q * NSL, a[i][q], c[i][RL - 1 - q], tw);
+                for (int q = 0; q < RL; ++q)
+                {
+                    cpx<T> x0, x1;
+                    r2c_val(a[i][q], c[i][RL - 1 - q], split_twiddle<T, L, RL>(wj, j, q, tw), x0, x1);
+                    stor<SO>::st(orow + 2 * (j + q * NSL), x0);
+                    stor<SO>::st(orow + 2 * (L - j - q * NSL), x1);
+                }
             }
         }
         if (self)
@@ -322,13 +362,7 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
         for (int i = 0; i < B::NMAIN; ++i)
         {
             const int j = t + i * NT;
-            const int jp = NSL - j;
-#pragma unroll
-            for (int r = 1; r < RL; ++r)
-            {
-                a[i][r] = cmul(a[i][r], tw[2 * r * j]);
-                c[i][r] = cmul(c[i][r], tw[(2 * r * jp) & (2 * L - 1)]);
-            }
+            fused_twiddles<T, L, RL>(a[i], c[i], j, tw);
             dftR<T, RL>(a[i]);
             dftR<T, RL>(c[i]);
             if (j == 0)
@@ -346,8 +380,10 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
             }
             else
             {
+                const cpx<T> wj = tw[j];
 #pragma unroll
-                for (int q = 0; q < RL; ++q) r2c_val(a[i][q], c[i][RL - 1 - q], tw[j + q * NSL], xa[i][q], xb[i][q]);
+                for (int q = 0; q < RL; ++q)
+                    r2c_val(a[i][q], c[i][RL - 1 - q], split_twiddle<T, L, RL>(wj, j, q, tw), xa[i][q], xb[i][q]);
             }
 #pragma unroll
             for (int q = 0; q < RL; ++q) m = fmaxf(m, cmax(xa[i][q], xb[i][q]));
