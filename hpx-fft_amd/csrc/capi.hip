@@ -377,16 +377,22 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
     a.cexp_dist = (long long)p->ncols;
     if (p->gen_col)
     {
-        // general length (forward, in place on spec_out = spec_in)
+        // general length (F64 / F32; in place for the forward transform)
+        a.in = spec_in;
         a.out = spec_out;
         const dim3 grid((unsigned)((p->ncols + p->gen_w - 1) / p->gen_w), (unsigned)p->batch);
+        const dim3 block(hpxfft_dev::GEN_NT);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab) + p->gtab_N;
         if (p->prec == P_F64)
-            hipLaunchKernelGGL(hpxfft_dev::gen_col_kernel<double>, grid, dim3(hpxfft_dev::GEN_NT), 0, p->stream, a,
-                               p->gcol, tab, p->gen_w);
+        {
+            if (conj) hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<double, true>), grid, block, 0, p->stream, a, p->gcol, tab, p->gen_w);
+            else hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<double, false>), grid, block, 0, p->stream, a, p->gcol, tab, p->gen_w);
+        }
         else
-            hipLaunchKernelGGL(hpxfft_dev::gen_col_kernel<float>, grid, dim3(hpxfft_dev::GEN_NT), 0, p->stream, a,
-                               p->gcol, tab, p->gen_w);
+        {
+            if (conj) hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<float, true>), grid, block, 0, p->stream, a, p->gcol, tab, p->gen_w);
+            else hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<float, false>), grid, block, 0, p->stream, a, p->gcol, tab, p->gen_w);
+        }
         HIPCHK(hipGetLastError());
         return HPXFFT_OK;
     }
@@ -825,8 +831,6 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     if (!d_spec || !d_real) return fail(HPXFFT_INVALID_ARG, "null device pointer");
     if (plan->prec != P_F64 && plan->prec != P_F32)
         return fail(HPXFFT_INVALID_ARG, std::string(what) + " is implemented for F64 and F32 plans");
-    if (plan->gen_row || plan->gen_col)
-        return fail(HPXFFT_UNSUPPORTED_SIZE, std::string(what) + " is implemented for power-of-two Nx and Ny");
     device_guard g(plan->device);
     const size_t sb = storage_bytes(plan->prec);
     const size_t bytes = plan->batch * plan->dist_out * 2 * sb;
@@ -849,9 +853,29 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     r.nrows = (int)plan->nx;
     r.tw = tw_ptr(plan, plan->ny);
     r.tw64 = tw64_ptr(plan, plan->ny);
-    const geom rg = row_geom(plan->prec, plan->lgL);
-    const unsigned gx = (unsigned)((plan->nx + rg.nseq - 1) / rg.nseq);
-    HIPCHK(row_c2r(plan->prec, plan->lgL, halve, r, gx, (unsigned)plan->batch, plan->stream));
+    if (plan->gen_row)
+    {
+        const dim3 grid((unsigned)plan->nx, (unsigned)plan->batch), block(hpxfft_dev::GEN_NT);
+        const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(plan->gen_tab);
+        const auto *tL = tab + plan->gtab_L, *t2 = tab + plan->gtab_2;
+        if (plan->prec == P_F64)
+        {
+            if (halve) hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<double, true>), grid, block, 0, plan->stream, r, plan->grow, tL, t2);
+            else hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<double, false>), grid, block, 0, plan->stream, r, plan->grow, tL, t2);
+        }
+        else
+        {
+            if (halve) hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<float, true>), grid, block, 0, plan->stream, r, plan->grow, tL, t2);
+            else hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<float, false>), grid, block, 0, plan->stream, r, plan->grow, tL, t2);
+        }
+        HIPCHK(hipGetLastError());
+    }
+    else
+    {
+        const geom rg = row_geom(plan->prec, plan->lgL);
+        const unsigned gx = (unsigned)((plan->nx + rg.nseq - 1) / rg.nseq);
+        HIPCHK(row_c2r(plan->prec, plan->lgL, halve, r, gx, (unsigned)plan->batch, plan->stream));
+    }
     HIPCHK(hipEventRecord(plan->ev[2], plan->stream));
     plan->ran = true;
     plan->col_split = false;

@@ -134,10 +134,11 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
     }
 }
 
-// Column c2c of any Nx, in place on the spectrum: a workgroup takes W
-// adjacent columns over all Nx rows (W * Nx <= gen_max_points).  blockIdx.x
-// = strip, .y = grid.  tab: w_Nx^m (m < Nx).
-template <typename T>
+// Column c2c of any Nx: a workgroup takes W adjacent columns over all Nx rows
+// (W * Nx <= gen_max_points), from a.in to a.out (the same spectrum for the
+// forward transform, in place).  CONJ: the inverse transform, computed as
+// conj(DFT(conj .)).  blockIdx.x = strip, .y = grid.  tab: w_Nx^m (m < Nx).
+template <typename T, bool CONJ>
 __global__ void __launch_bounds__(GEN_NT) gen_col_kernel(ColArgs a, GenPlan g, const cpx<double> *tab, int W)
 {
     constexpr int CAP = gen_max_points<T>();
@@ -147,19 +148,68 @@ __global__ void __launch_bounds__(GEN_NT) gen_col_kernel(ColArgs a, GenPlan g, c
     const long long b = blockIdx.y;
     const int col0 = blockIdx.x * W;
     const int nvalid = min(W, a.ncols - col0);
-    T *spec = reinterpret_cast<T *>(a.out) + 2 * b * a.dist;
+    const T *src = reinterpret_cast<const T *>(a.in) + 2 * b * a.dist;
+    T *dst = reinterpret_cast<T *>(a.out) + 2 * b * a.dist;
     // load row segments: element (e, w) -> buf[w * n + e]
     for (int i = t; i < n * W; i += GEN_NT)
     {
         const int e = i / W, w = i - e * W;
-        buf[w * n + e] = w < nvalid ? stor<T>::ld(spec + 2 * ((long long)e * a.pitch + col0 + w)) : cpx<T>{T(0), T(0)};
+        cpx<T> z = {T(0), T(0)};
+        if (w < nvalid) z = stor<T>::ld(src + 2 * ((long long)e * a.pitch + col0 + w));
+        buf[w * n + e] = CONJ ? conjc(z) : z;
     }
     __syncthreads();
     const cpx<T> *X = gen_fft<T>(buf, buf + CAP, roots, W, g, tab);
     for (int i = t; i < n * W; i += GEN_NT)
     {
         const int e = i / W, w = i - e * W;
-        if (w < nvalid) stor<T>::st(spec + 2 * ((long long)e * a.pitch + col0 + w), X[w * n + e]);
+        const cpx<T> z = X[w * n + e];
+        if (w < nvalid) stor<T>::st(dst + 2 * ((long long)e * a.pitch + col0 + w), CONJ ? conjc(z) : z);
+    }
+}
+
+// Row c2r of any even Ny (the row half of the adjoint when HALVE, the
+// unnormalised inverse c2r otherwise): half-spectrum row G[0..L] in, Ny reals out,
+//   H = G on 0<k<L (halved when HALVE), H[0] = Re G[0], H[L] = Re G[L],
+//   Z[k] = (H[k] + conj H[L-k]) + i (H[k] - conj H[L-k]) conj(w_{Ny}^k),
+//   out[2n] + i out[2n+1] = IDFT_L(Z) = conj(DFT(conj Z)).
+template <typename T, bool HALVE>
+__global__ void __launch_bounds__(GEN_NT) gen_row_c2r_kernel(RowArgs a, GenPlan g, const cpx<double> *tabL,
+                                                            const cpx<double> *tab2)
+{
+    constexpr int CAP = gen_max_points<T>();
+    __shared__ cpx<T> buf[2 * CAP];
+    __shared__ cpx<T> roots[GEN_MAX_RADIX];
+    const int t = threadIdx.x, L = g.n;
+    const long long b = blockIdx.y;
+    const int row = blockIdx.x;
+    const T *in = reinterpret_cast<const T *>(a.in) + b * a.dist_in + (long long)row * a.ld_in;
+    T *out = reinterpret_cast<T *>(a.out) + b * a.dist_out + (long long)row * a.ld_out;
+    const T hs = HALVE ? T(0.5) : T(1);
+    for (int k = t; k < L; k += GEN_NT)
+    {
+        cpx<T> z;
+        if (k == 0)
+        {
+            const T h0 = stor<T>::ld(in).x, hl = stor<T>::ld(in + 2 * L).x;
+            z = cpx<T>{h0 + hl, h0 - hl};
+        }
+        else
+        {
+            const cpx<T> hk = scal(stor<T>::ld(in + 2 * k), hs), hl = scal(stor<T>::ld(in + 2 * (L - k)), hs);
+            const cpx<double> w64 = tab2[k];
+            const cpx<T> wc = {(T)w64.x, -(T)w64.y};
+            const cpx<T> s0 = hk + conjc(hl), d0 = cmul(hk - conjc(hl), wc);
+            z = cpx<T>{s0.x - d0.y, s0.y + d0.x};
+        }
+        buf[k] = conjc(z);
+    }
+    __syncthreads();
+    const cpx<T> *U = gen_fft<T>(buf, buf + CAP, roots, 1, g, tabL);
+    for (int e = t; e < L; e += GEN_NT)
+    {
+        const cpx<T> u = U[e];
+        stor<T>::st(out + 2 * e, cpx<T>{u.x, -u.y});
     }
 }
 

@@ -5,8 +5,8 @@ kernels cover lengths with prime factors <= 61 that fit one workgroup's LDS.
 Checks against the oracle (itself pinned to brute force and numpy for these
 lengths): FP64 <= 1e-12, FP32 <= 1e-5 relative L2, with general rows and/or
 general columns combined with every power-of-two kernel family; batches;
-the reference harness at its default 8 x 14; and that the adjoint reports
-these sizes as unsupported.
+the reference harness at its default 8 x 14; and
+the backward forms (adjoint, inverse c2r, autograd) on these lengths.
 """
 import os
 import subprocess
@@ -73,12 +73,43 @@ def test_general_batched(gpu):
         assert oracle.rel_l2(g, oracle.r2c_2d(x, ny)) <= 1e-12, b
 
 
-def test_adjoint_reports_general_lengths_unsupported(gpu):
+BACK_SIZES = [(8, 14), (5, 6), (1, 6), (12, 100), (1000, 64), (64, 1000), (4096, 30), (10, 4096), (3, 2)]
+
+
+@pytest.mark.parametrize("nx,ny", BACK_SIZES)
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5)])
+def test_general_adjoint_and_c2r(gpu, nx, ny, prec, tol):
+    """The backward forms on general lengths: adjoint (interior halved) and
+    the unnormalised inverse c2r, against the oracle's FP64 versions."""
     import torch
     import hpxfft_amd
-    Y = torch.zeros((8, 8), dtype=torch.complex128, device="cuda")
-    with pytest.raises(hpxfft_amd.HpxfftError, match="power-of-two"):
-        hpxfft_amd.rfft2_adjoint(Y, 14)
+    print(f"general backward {prec} {nx}x{ny}", flush=True)
+    rng = np.random.default_rng(nx * 7 + ny)
+    Y = rng.standard_normal((nx, ny // 2 + 1)) + 1j * rng.standard_normal((nx, ny // 2 + 1))
+    cdt = torch.complex128 if prec == "f64" else torch.complex64
+    Yd = torch.from_numpy(Y).to(device="cuda", dtype=cdt)
+    Yr = Yd.cpu().numpy().astype(np.complex128)   # the same (rounded) input
+    adj = hpxfft_amd.rfft2_adjoint(Yd, ny).cpu().numpy().astype(np.float64)
+    assert oracle.rel_l2(adj, oracle.adjoint_r2c_2d(Yr, ny)) <= tol
+    inv = hpxfft_amd.c2r_2d(Yd, ny).cpu().numpy().astype(np.float64)
+    assert oracle.rel_l2(inv, oracle.c2r_2d(Yr, ny)) <= tol
+
+
+def test_general_autograd_and_round_trip(gpu):
+    """rfft2_autograd on 8 x 14 against torch.fft FP64 autograd on the CPU, and
+    c2r(r2c(x)) = Nx Ny x."""
+    import torch
+    import hpxfft_amd
+    nx, ny = 8, 14
+    xc = torch.randn(nx, ny, dtype=torch.float64)
+    w = torch.rand(nx, ny // 2 + 1, dtype=torch.float64)
+    xr = xc.clone().requires_grad_(True)
+    torch.sum(w * torch.abs(torch.fft.rfft2(xr)) ** 2).backward()
+    xg = xc.cuda().requires_grad_(True)
+    torch.sum(w.cuda() * torch.abs(hpxfft_amd.rfft2_autograd(xg)) ** 2).backward()
+    assert oracle.rel_l2(xg.grad.cpu().numpy(), xr.grad.numpy()) <= 1e-12
+    back = hpxfft_amd.c2r_2d(hpxfft_amd.rfft2(xc.cuda()), ny).cpu()
+    assert torch.allclose(back / (nx * ny), xc, rtol=0, atol=1e-13)
 
 
 def test_cpp_harness_default_size(gpu, tmp_path):
