@@ -2,8 +2,12 @@
 // driving hpxfft::shared::loop, here resolved to the MI355X class.
 //
 // Same command line (shared_loop_2d.cpp:140-146): --nx (default 8) --ny
-// (default 14) --plan (estimate) --run (par|seq) --header (0) --result (0),
-// given as --key=value or --key value.  Same ramp input x[i][j] = j (:33-40),
+// (default 14) --plan (estimate) --run (par|seq) --header (false) --result
+// (false), given as --key=value or --key value; the two flags take the words
+// a program_options bool takes (true/false, 1/0, yes/no, on/off, any case),
+// as in the reference's size sweep (benchmark/sbatch_scripts/
+// run_hpxfft_size_shared.sh:28, --header=true).  An unknown option or a value
+// that does not parse is a usage error: a message and exit status 2.  Same ramp input x[i][j] = j (:33-40),
 // same console report (:69-93), same ';'-separated CSV appended to
 // runtimes/runtimes_hpx_shared_loop.txt (:95-111), same plan file
 // plans/plan_hpx_shared_loop.txt (:114-128).  The n_threads column holds the
@@ -13,6 +17,7 @@
 // exits with status 2.
 #include "hpxfft/shared/loop.hpp"
 
+#include <cctype>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -20,12 +25,40 @@
 #include <fstream>
 #include <iostream>
 #include <map>
+#include <stdexcept>
 #include <string>
 
 static void create_parent_dir(const std::string &path)
 {
     const std::filesystem::path p(path);
     if (p.has_parent_path()) std::filesystem::create_directories(p.parent_path());
+}
+
+// program_options' bool words; throws std::invalid_argument otherwise.
+static bool parse_flag(const std::string &key, std::string v)
+{
+    for (auto &c : v) c = (char)std::tolower((unsigned char)c);
+    if (v == "1" || v == "true" || v == "yes" || v == "on") return true;
+    if (v == "0" || v == "false" || v == "no" || v == "off") return false;
+    throw std::invalid_argument("the argument ('" + v + "') for option '--" + key + "' is invalid");
+}
+
+static std::size_t parse_size(const std::string &key, const std::string &v)
+{
+    std::size_t used = 0;
+    unsigned long long x = 0;
+    try
+    {
+        if (v.empty() || v[0] == '-') throw std::invalid_argument(v);
+        x = std::stoull(v, &used);
+    }
+    catch (const std::exception &)
+    {
+        used = 0;
+    }
+    if (used == 0 || used != v.size())
+        throw std::invalid_argument("the argument ('" + v + "') for option '--" + key + "' is invalid");
+    return (std::size_t)x;
 }
 
 static void print_vector_2d(const hpxfft::shared::vector_2d &v)
@@ -46,21 +79,40 @@ static void print_vector_2d(const hpxfft::shared::vector_2d &v)
 
 int main(int argc, char *argv[])
 {
-    std::map<std::string, std::string> opt = {{"result", "0"}, {"nx", "8"},      {"ny", "14"},
-                                              {"plan", "estimate"}, {"run", "par"}, {"header", "0"}};
-    for (int i = 1; i < argc; ++i)
+    std::map<std::string, std::string> opt = {{"result", "false"}, {"nx", "8"},      {"ny", "14"},
+                                              {"plan", "estimate"}, {"run", "par"}, {"header", "false"}};
+    bool print_result = false, print_header = false;
+    std::size_t dim_c_x = 0, dim_r_y = 0;
+    try
     {
-        std::string a = argv[i];
-        if (a.rfind("--", 0) != 0) continue;
-        a = a.substr(2);
-        const auto eq = a.find('=');
-        if (eq != std::string::npos) opt[a.substr(0, eq)] = a.substr(eq + 1);
-        else if (i + 1 < argc) opt[a] = argv[++i];
+        for (int i = 1; i < argc; ++i)
+        {
+            std::string a = argv[i];
+            if (a.rfind("--", 0) != 0) throw std::invalid_argument("unexpected argument '" + a + "'");
+            a = a.substr(2);
+            const auto eq = a.find('=');
+            std::string key = a, val;
+            if (eq != std::string::npos)
+            {
+                key = a.substr(0, eq);
+                val = a.substr(eq + 1);
+            }
+            else if (i + 1 < argc) val = argv[++i];
+            else throw std::invalid_argument("option '--" + key + "' requires an argument");
+            if (!opt.count(key)) throw std::invalid_argument("unrecognised option '--" + key + "'");
+            opt[key] = val;
+        }
+        print_result = parse_flag("result", opt["result"]);
+        print_header = parse_flag("header", opt["header"]);
+        dim_c_x = parse_size("nx", opt["nx"]);
+        dim_r_y = parse_size("ny", opt["ny"]);
+    }
+    catch (const std::invalid_argument &e)
+    {
+        std::cerr << "usage error: " << e.what() << std::endl;
+        return 2;
     }
     const std::string run_flag = opt["run"], plan_flag = opt["plan"];
-    const bool print_result = std::stoi(opt["result"]) != 0, print_header = std::stoi(opt["header"]) != 0;
-    const std::size_t dim_c_x = std::stoul(opt["nx"]);
-    const std::size_t dim_r_y = std::stoul(opt["ny"]);
     const std::size_t dim_c_y = dim_r_y / 2 + 1;
 
     using clock = std::chrono::steady_clock;

@@ -161,6 +161,66 @@ def test_cpp_harness_argument_errors(tmp_path):
     assert r.returncode == 2 and "prime factors" in r.stderr       # Ny/2 = 67
 
 
+@pytest.mark.parametrize("args,msg", [
+    (["--header=maybe"], "('maybe') for option '--header' is invalid"),
+    (["--result", "2"], "('2') for option '--result' is invalid"),
+    (["--nx=12x"], "('12x') for option '--nx' is invalid"),
+    (["--ny=-4"], "('-4') for option '--ny' is invalid"),
+    (["--threads=4"], "unrecognised option '--threads'"),
+    (["--nx"], "option '--nx' requires an argument"),
+    (["8"], "unexpected argument '8'"),
+])
+def test_cpp_harness_usage_errors(tmp_path, args, msg):
+    """Command-line values are checked as program_options checks them
+    (shared_loop_2d.cpp:140-146): a usage error, exit status 2, before any
+    device work and before any file is written."""
+    r = subprocess.run([os.path.join(BIN, "shared_loop_2d_gpu")] + args, capture_output=True, text=True,
+                       timeout=60, cwd=tmp_path)
+    assert r.returncode == 2 and msg in r.stderr, r.stderr
+    assert not (tmp_path / "runtimes").exists()
+
+
+@pytest.mark.parametrize("word", ["true", "TRUE", "yes", "on", "1", "false", "No", "off", "0"])
+def test_cpp_harness_bool_words(tmp_path, word):
+    """--header / --result take program_options' bool words, as the
+    reference's size sweep passes --header=true.  The unsupported size makes
+    the run stop after argument parsing, at plan validation (status 2)."""
+    r = subprocess.run([os.path.join(BIN, "shared_loop_2d_gpu"), "--header=" + word, "--result", word, "--nx=8",
+                        "--ny=134"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert r.returncode == 2 and "prime factors" in r.stderr and "usage error" not in r.stderr, r.stderr
+
+
+SWEEP = os.path.join(REPO, "hpx-fft_amd", "cpp", "run_size_sweep.sh")
+
+
+def test_size_sweep_script_drives_the_harness(tmp_path):
+    """The sweep (argument order of run_hpxfft_size_shared.sh:10-15) with a
+    stand-in executable that records its arguments: sizes 2^3..2^5, two runs
+    each, --header=true on the first run only."""
+    stub = tmp_path / "stub.sh"
+    stub.write_text("#!/bin/bash\necho \"$@\" >> calls.txt\n")
+    stub.chmod(0o755)
+    r = subprocess.run(["bash", SWEEP, str(stub), "measure", "64", "3", "5", "2"], capture_output=True, text=True,
+                       timeout=60, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    calls = (tmp_path / "calls.txt").read_text().splitlines()
+    want = [f"--nx={n} --ny={n} --plan=measure --header={'true' if i == 0 else 'false'}"
+            for i, n in enumerate([8, 8, 16, 16, 32, 32])]
+    assert calls == want
+
+
+def test_size_sweep_script_stops_at_a_failed_run(tmp_path):
+    stub = tmp_path / "stub.sh"
+    stub.write_text("#!/bin/bash\necho \"$@\" >> calls.txt\ncase \"$1\" in --nx=16) exit 2;; esac\n")
+    stub.chmod(0o755)
+    r = subprocess.run(["bash", SWEEP, str(stub), "estimate", "1", "3", "6", "1"], capture_output=True, text=True,
+                       timeout=60, cwd=tmp_path)
+    assert r.returncode == 2 and "nx=ny=16" in r.stderr
+    assert len((tmp_path / "calls.txt").read_text().splitlines()) == 2
+    r = subprocess.run(["bash", SWEEP, str(stub)], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
 def test_cpp_known_answer_without_device():
     st = plan_status(4, 4)
     if st == _lib.OK:

@@ -106,3 +106,17 @@ def test_cpp_harness(gpu, tmp_path):
                        text=True, timeout=120, cwd=tmp_path)
     # ramp input x[i][j] = j on 4 x 4: X[0] = (24,0) (-8,8) (-8,0), other rows 0
     assert r.returncode == 0 and "(24 0) (-8 8) (-8 0)" in r.stdout, r.stdout
+
+
+def test_size_sweep_with_harness(gpu, tmp_path):
+    """The size sweep script over 2^3..2^6 grids, two runs each, on the real
+    harness: one CSV header (--header=true) then eight rows, sizes in order."""
+    sweep = os.path.join(REPO, "hpx-fft_amd", "cpp", "run_size_sweep.sh")
+    exe = os.path.join(BIN, "shared_loop_2d_gpu")
+    r = subprocess.run(["bash", sweep, exe, "estimate", "1", "3", "6", "2"], capture_output=True, text=True,
+                       timeout=120, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = (tmp_path / "runtimes" / "runtimes_hpx_shared_loop.txt").read_text().splitlines()
+    assert rows[0].startswith("n_threads;n_x;n_y;") and len(rows) == 9
+    assert [int(x.split(";")[1]) for x in rows[1:]] == [8, 8, 16, 16, 32, 32, 64, 64]
+    assert all(float(x.split(";")[7]) > 0 for x in rows[1:])   # fft_2d_total
