@@ -191,6 +191,7 @@ struct hpxfft_plan_s
     bool gen_row = false, gen_col = false;
     hpxfft_dev::GenPlan grow{}, gcol{};
     int gen_w = 1;                 // general column kernel: adjacent columns per workgroup
+    int gen_rows = 1;              // general row kernels: rows per workgroup
     double *gen_tab = nullptr;     // exact FP64 tables w_n^m: rows L = Ny/2, split Ny, columns Nx
     size_t gtab_L = 0, gtab_2 = 0, gtab_N = 0;   // their offsets (complex entries)
     // device memory
@@ -235,6 +236,29 @@ const void *tw_ptr(const hpxfft_plan_s *p, size_t N)
     return p->tabs.tw32 + 2 * N;
 }
 const void *tw64_ptr(const hpxfft_plan_s *p, size_t N) { return p->tabs.tw64 + 2 * N; }
+
+// dynamic LDS of a general kernel: two buffers of `points` complex values in the compute type
+size_t gen_lds_bytes(const hpxfft_plan_s *p, size_t points) { return 2 * points * 2 * compute_bytes(p->prec); }
+
+// let every general kernel take up to GEN_LDS_BYTES of dynamic LDS (above the
+// 64 KiB default); once per device
+int gen_allow_lds(int dev)
+{
+    static std::mutex mu;
+    static std::map<int, bool> done;
+    std::lock_guard<std::mutex> lk(mu);
+    if (done[dev]) return HPXFFT_OK;
+    using namespace hpxfft_dev;
+    const void *fns[] = {
+        reinterpret_cast<const void *>(&gen_row_r2c_kernel<double>), reinterpret_cast<const void *>(&gen_row_r2c_kernel<float>),
+        reinterpret_cast<const void *>(&gen_col_kernel<double, false>), reinterpret_cast<const void *>(&gen_col_kernel<double, true>),
+        reinterpret_cast<const void *>(&gen_col_kernel<float, false>), reinterpret_cast<const void *>(&gen_col_kernel<float, true>),
+        reinterpret_cast<const void *>(&gen_row_c2r_kernel<double, false>), reinterpret_cast<const void *>(&gen_row_c2r_kernel<double, true>),
+        reinterpret_cast<const void *>(&gen_row_c2r_kernel<float, false>), reinterpret_cast<const void *>(&gen_row_c2r_kernel<float, true>)};
+    for (const void *f : fns) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, GEN_LDS_BYTES));
+    done[dev] = true;
+    return HPXFFT_OK;
+}
 
 // Stockham radices for a length that is not a power of two: 4s, then a 2,
 // then odd primes ascending; false if a prime factor exceeds GEN_MAX_RADIX
@@ -301,7 +325,8 @@ std::string describe(const hpxfft_plan_s *p)
        << prec_name(p->prec) << " effort " << p->effort << "\n";
     if (p->gen_row)
         os << "  row r2c kernel (general length): " << p->ny / 2 << "-point mixed-radix Stockham in LDS, radices "
-           << radices(p->grow) << ", 1 row/workgroup, " << hpxfft_dev::GEN_NT << " threads + r2c split\n";
+           << radices(p->grow) << ", " << p->gen_rows << " row(s)/workgroup, " << hpxfft_dev::GEN_NT
+           << " threads + r2c split\n";
     else
     {
         const geom rg = row_geom(p->prec, p->lgL);
@@ -382,16 +407,17 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         a.out = spec_out;
         const dim3 grid((unsigned)((p->ncols + p->gen_w - 1) / p->gen_w), (unsigned)p->batch);
         const dim3 block(hpxfft_dev::GEN_NT);
+        const size_t lds = gen_lds_bytes(p, (size_t)p->gen_w * p->nx);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab) + p->gtab_N;
         if (p->prec == P_F64)
         {
-            if (conj) hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<double, true>), grid, block, 0, p->stream, a, p->gcol, tab, p->gen_w);
-            else hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<double, false>), grid, block, 0, p->stream, a, p->gcol, tab, p->gen_w);
+            if (conj) hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<double, true>), grid, block, lds, p->stream, a, p->gcol, tab, p->gen_w);
+            else hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<double, false>), grid, block, lds, p->stream, a, p->gcol, tab, p->gen_w);
         }
         else
         {
-            if (conj) hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<float, true>), grid, block, 0, p->stream, a, p->gcol, tab, p->gen_w);
-            else hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<float, false>), grid, block, 0, p->stream, a, p->gcol, tab, p->gen_w);
+            if (conj) hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<float, true>), grid, block, lds, p->stream, a, p->gcol, tab, p->gen_w);
+            else hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<float, false>), grid, block, lds, p->stream, a, p->gcol, tab, p->gen_w);
         }
         HIPCHK(hipGetLastError());
         return HPXFFT_OK;
@@ -469,14 +495,16 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     r.exp_dist = (long long)p->nx;
     if (p->gen_row)
     {
-        const dim3 grid((unsigned)p->nx, (unsigned)p->batch);
+        const int ns = p->gen_rows;
+        const dim3 grid((unsigned)((p->nx + ns - 1) / ns), (unsigned)p->batch);
+        const size_t lds = gen_lds_bytes(p, (size_t)ns * (p->ny / 2));
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
         if (p->prec == P_F64)
-            hipLaunchKernelGGL(hpxfft_dev::gen_row_r2c_kernel<double>, grid, dim3(hpxfft_dev::GEN_NT), 0, p->stream,
-                               r, p->grow, tab + p->gtab_L, tab + p->gtab_2);
+            hipLaunchKernelGGL(hpxfft_dev::gen_row_r2c_kernel<double>, grid, dim3(hpxfft_dev::GEN_NT), lds, p->stream,
+                               r, p->grow, tab + p->gtab_L, tab + p->gtab_2, ns);
         else
-            hipLaunchKernelGGL(hpxfft_dev::gen_row_r2c_kernel<float>, grid, dim3(hpxfft_dev::GEN_NT), 0, p->stream,
-                               r, p->grow, tab + p->gtab_L, tab + p->gtab_2);
+            hipLaunchKernelGGL(hpxfft_dev::gen_row_r2c_kernel<float>, grid, dim3(hpxfft_dev::GEN_NT), lds, p->stream,
+                               r, p->grow, tab + p->gtab_L, tab + p->gtab_2, ns);
         HIPCHK(hipGetLastError());
     }
     else
@@ -574,6 +602,12 @@ int plan_common(hpxfft_plan_s *p)
                                                      " needs prime factors <= " +
                                                      std::to_string(hpxfft_dev::GEN_MAX_RADIX) + " and at most " +
                                                      std::to_string(cap) + " points (got " + dims + ")");
+        if (p->gen_row)
+        {
+            // short rows share a workgroup: about 2048 points per workgroup
+            const size_t L = p->ny / 2;
+            p->gen_rows = (int)std::max<size_t>(1, std::min(cap / L, std::min<size_t>(p->nx, 2048 / L)));
+        }
         if (p->gen_col)
         {
             // one 128-byte row segment per workgroup where the LDS allows
@@ -597,6 +631,8 @@ int plan_common(hpxfft_plan_s *p)
     if (p->dist_in % 2 != 0) return fail(HPXFFT_INVALID_ARG, "dist_in must be even");
     if (int rc = check_device(p->device)) return rc;
     if (int rc = get_tables(p->device, p->tabs)) return rc;
+    if (p->gen_row || p->gen_col)
+        if (int rc = gen_allow_lds(p->device)) return rc;
     if (p->gen_row || p->gen_col)
     {
         // exact FP64 tables (the long-double evaluation of the power-of-two tables)
@@ -855,18 +891,20 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     r.tw64 = tw64_ptr(plan, plan->ny);
     if (plan->gen_row)
     {
-        const dim3 grid((unsigned)plan->nx, (unsigned)plan->batch), block(hpxfft_dev::GEN_NT);
+        const int ns = plan->gen_rows;
+        const dim3 grid((unsigned)((plan->nx + ns - 1) / ns), (unsigned)plan->batch), block(hpxfft_dev::GEN_NT);
+        const size_t lds = gen_lds_bytes(plan, (size_t)ns * (plan->ny / 2));
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(plan->gen_tab);
         const auto *tL = tab + plan->gtab_L, *t2 = tab + plan->gtab_2;
         if (plan->prec == P_F64)
         {
-            if (halve) hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<double, true>), grid, block, 0, plan->stream, r, plan->grow, tL, t2);
-            else hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<double, false>), grid, block, 0, plan->stream, r, plan->grow, tL, t2);
+            if (halve) hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<double, true>), grid, block, lds, plan->stream, r, plan->grow, tL, t2, ns);
+            else hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<double, false>), grid, block, lds, plan->stream, r, plan->grow, tL, t2, ns);
         }
         else
         {
-            if (halve) hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<float, true>), grid, block, 0, plan->stream, r, plan->grow, tL, t2);
-            else hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<float, false>), grid, block, 0, plan->stream, r, plan->grow, tL, t2);
+            if (halve) hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<float, true>), grid, block, lds, plan->stream, r, plan->grow, tL, t2, ns);
+            else hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<float, false>), grid, block, lds, plan->stream, r, plan->grow, tL, t2, ns);
         }
         HIPCHK(hipGetLastError());
     }

@@ -2,17 +2,19 @@
 // (FP64 / FP32).  The reference accepts any even Ny and any Nx through FFTW
 // (core/src/util/adapter_fftw.cpp:9,28-29); these kernels cover the lengths
 // whose prime factors are at most GEN_MAX_RADIX and whose data fit one
-// workgroup's LDS, with the power-of-two kernels kept for power-of-two
-// dimensions.
+// workgroup's LDS buffer, with the power-of-two kernels kept for
+// power-of-two dimensions.
 //
-// One workgroup transforms NSEQ sequences of length n held in LDS, by
-// Stockham autosort stages of runtime radices r (product n):
+// One workgroup transforms NSEQ sequences of length n held in LDS (sequence
+// s at [s*n, (s+1)*n), two ping-pong buffers in dynamic LDS sized to the
+// plan), by Stockham autosort stages of runtime radices r (product n):
 //   stage with sub-length Ns, butterfly j < n/r, k = j mod Ns:
 //     x_q = A[j + q n/r] w_{r Ns}^{q k},  y = DFT_r(x),
 //     B[(j / Ns) r Ns + k + q Ns] = y_q
-// Stage twiddles come from the plan's exact table w_n^m (m < n); the radix-r
-// DFT uses the r roots w_r^e = w_n^{e n / r} staged in LDS.  Sequences are
-// stored sequence-major in LDS (seq * n + e).
+// Radices 2, 3, 4, 5 and 7 run as register butterflies (one LDS pass per
+// stage); larger primes (up to 61) by a generic stage of two passes.
+// Twiddles w_n^m and the roots w_r^e = w_n^{e n/r} come from the plan's
+// exact FP64 table (rounded once for FP32).
 #pragma once
 
 #include "fft_kernels.hpp"
@@ -21,9 +23,8 @@ namespace hpxfft_dev {
 
 constexpr int GEN_MAX_STAGES = 24;
 constexpr int GEN_MAX_RADIX = 61;
-constexpr int GEN_NT = 256;   // threads per workgroup
-// LDS holds two buffers of up to GEN_LDS_BYTES / 2 each
-constexpr int GEN_LDS_BYTES = 128 * 1024;
+constexpr int GEN_NT = 256;                  // threads per workgroup
+constexpr int GEN_LDS_BYTES = 128 * 1024;    // the largest pair of sequence buffers of a workgroup
 template <typename T> constexpr int gen_max_points() { return GEN_LDS_BYTES / (2 * 2 * (int)sizeof(T)); }
 
 struct GenPlan
@@ -33,53 +34,144 @@ struct GenPlan
     int radix[GEN_MAX_STAGES];      // stage radices, product n
 };
 
-// Transform the NSEQ sequences in A (ping-pong with B); returns the buffer
-// holding the result.  tab: w_n^m, m < n (forward sign).  Each stage runs as
-// two LDS passes: twiddle the inputs in place, then every output reads the
-// r inputs of its butterfly (no per-thread arrays of runtime size).
-template <typename T>
-__device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, cpx<T> *roots, int nseq, const GenPlan &g,
-                           const cpx<double> *__restrict__ tab)
+// x / d for 0 <= x < 2^20 and d >= 1, from inv = 1/d in FP32: the product is
+// within 2^-21 * x of x/d, so the truncated quotient is off by at most one,
+// which the remainder test corrects
+__device__ __forceinline__ int gdiv(int x, int d, float inv)
 {
-    const int t = threadIdx.x, n = g.n, total = nseq * n;
+    int q = (int)((float)x * inv);
+    const int r = x - q * d;
+    if (r < 0) --q;
+    else if (r >= d) ++q;
+    return q;
+}
+
+template <typename T> __device__ __forceinline__ cpx<T> tabv(const cpx<double> *__restrict__ tab, int m)
+{
+    const cpx<double> w = tab[m];
+    return cpx<T>{(T)w.x, (T)w.y};
+}
+
+// DFT of R points in registers, forward sign; w[e] = w_R^e (e >= 1)
+template <typename T, int R> __device__ __forceinline__ void dft_small(cpx<T> *x, const cpx<T> *w)
+{
+    if constexpr (R == 2)
+    {
+        const cpx<T> a = x[0], b = x[1];
+        x[0] = a + b;
+        x[1] = a - b;
+    }
+    else if constexpr (R == 4)
+    {
+        const cpx<T> s0 = x[0] + x[2], d0 = x[0] - x[2], s1 = x[1] + x[3], d1 = x[1] - x[3];
+        x[0] = s0 + s1;
+        x[2] = s0 - s1;
+        x[1] = d0 + mul_mi(d1);
+        x[3] = d0 - mul_mi(d1);
+    }
+    else
+    {
+        cpx<T> y[R];
+#pragma unroll
+        for (int o = 0; o < R; ++o)
+        {
+            cpx<T> acc = x[0];
+#pragma unroll
+            for (int q = 1; q < R; ++q)
+            {
+                const int e = (q * o) % R;
+                acc = e == 0 ? acc + x[q] : acc + cmul(x[q], w[e]);
+            }
+            y[o] = acc;
+        }
+#pragma unroll
+        for (int o = 0; o < R; ++o) x[o] = y[o];
+    }
+}
+
+// one Stockham stage of radix R (2, 3, 4, 5, 7) over nseq sequences, A -> B
+template <typename T, int R>
+__device__ void gen_stage_reg(const cpx<T> *A, cpx<T> *B, int nseq, int n, int Ns, const cpx<double> *__restrict__ tab)
+{
+    const int t = threadIdx.x, m = n / R, nb = nseq * m, tstep = n / (R * Ns);
+    const float inv_m = 1.0f / (float)m, inv_ns = 1.0f / (float)Ns;
+    cpx<T> w[R];
+#pragma unroll
+    for (int e = 1; e < R; ++e) w[e] = tabv<T>(tab, e * m);
+    w[0] = cpx<T>{T(1), T(0)};
+    for (int b = t; b < nb; b += GEN_NT)
+    {
+        const int seq = gdiv(b, m, inv_m), j = b - seq * m;
+        const int jb = gdiv(j, Ns, inv_ns), k = j - jb * Ns;
+        const cpx<T> *src = A + seq * n + j;
+        cpx<T> x[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) x[q] = src[q * m];
+        if (k > 0)
+        {
+#pragma unroll
+            for (int q = 1; q < R; ++q) x[q] = cmul(x[q], tabv<T>(tab, q * k * tstep));
+        }
+        dft_small<T, R>(x, w);
+        cpx<T> *dst = B + seq * n + jb * R * Ns + k;
+#pragma unroll
+        for (int q = 0; q < R; ++q) dst[q * Ns] = x[q];
+    }
+    __syncthreads();
+}
+
+// one stage of a prime radix r > 7: twiddle A in place, then every output
+// reads the r inputs of its butterfly from A and goes to B
+template <typename T>
+__device__ void gen_stage_any(cpx<T> *A, cpx<T> *B, int nseq, int n, int r, int Ns, const cpx<double> *__restrict__ tab)
+{
+    const int t = threadIdx.x, m = n / r, total = nseq * n, tstep = n / (r * Ns);
+    const float inv_n = 1.0f / (float)n, inv_m = 1.0f / (float)m, inv_ns = 1.0f / (float)Ns;
+    for (int i = t; i < total; i += GEN_NT)
+    {
+        const int seq = gdiv(i, n, inv_n), idx = i - seq * n;
+        const int q = gdiv(idx, m, inv_m), j = idx - q * m;
+        const int k = j - gdiv(j, Ns, inv_ns) * Ns;
+        if (q > 0 && k > 0) A[i] = cmul(A[i], tabv<T>(tab, q * k * tstep));
+    }
+    __syncthreads();
+    for (int o = t; o < total; o += GEN_NT)
+    {
+        const int seq = gdiv(o, n, inv_n), oo = o - seq * n;
+        const int qo = gdiv(oo, m, inv_m), j = oo - qo * m;
+        const int jb = gdiv(j, Ns, inv_ns), k = j - jb * Ns;
+        const cpx<T> *src = A + seq * n + j;
+        cpx<T> acc = src[0];
+        int e = 0;
+        for (int q = 1; q < r; ++q)
+        {
+            e += qo;
+            if (e >= r) e -= r;
+            acc = acc + cmul(src[q * m], tabv<T>(tab, e * m));
+        }
+        B[seq * n + jb * r * Ns + k + qo * Ns] = acc;
+    }
+    __syncthreads();
+}
+
+// all stages of the plan on nseq sequences in A (B: the second buffer);
+// returns the buffer holding the result
+template <typename T>
+__device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
+{
     int Ns = 1;
     for (int s = 0; s < g.nstages; ++s)
     {
-        const int r = g.radix[s], m = n / r;
-        const long long tstep = n / (r * Ns);   // w_{r Ns}^{q k} = w_n^{q k tstep}, q k tstep < n
-        // roots of this radix: w_r^e = w_n^{e n / r}
-        for (int e = t; e < r; e += GEN_NT)
+        const int r = g.radix[s];
+        switch (r)
         {
-            const cpx<double> w = tab[(long long)e * m];
-            roots[e] = cpx<T>{(T)w.x, (T)w.y};
+        case 2: gen_stage_reg<T, 2>(A, B, nseq, g.n, Ns, tab); break;
+        case 3: gen_stage_reg<T, 3>(A, B, nseq, g.n, Ns, tab); break;
+        case 4: gen_stage_reg<T, 4>(A, B, nseq, g.n, Ns, tab); break;
+        case 5: gen_stage_reg<T, 5>(A, B, nseq, g.n, Ns, tab); break;
+        case 7: gen_stage_reg<T, 7>(A, B, nseq, g.n, Ns, tab); break;
+        default: gen_stage_any<T>(A, B, nseq, g.n, r, Ns, tab); break;
         }
-        // twiddle: element j + q m of a sequence times w_{r Ns}^{q (j mod Ns)}
-        for (int i = t; i < total; i += GEN_NT)
-        {
-            const int idx = i % n, q = idx / m, k = (idx - q * m) % Ns;
-            if (q > 0 && k > 0)
-            {
-                const cpx<double> w = tab[(long long)q * k * tstep];
-                A[i] = cmul(A[i], cpx<T>{(T)w.x, (T)w.y});
-            }
-        }
-        __syncthreads();
-        // radix-r DFTs: output qo of butterfly j -> B[(j / Ns) r Ns + (j mod Ns) + qo Ns]
-        for (int i = t; i < total; i += GEN_NT)
-        {
-            const int seq = i / n, o = i - seq * n, qo = o / m, j = o - qo * m, k = j % Ns;
-            const cpx<T> *src = A + (long long)seq * n + j;
-            cpx<T> acc = src[0];
-            int e = 0;
-            for (int q = 1; q < r; ++q)
-            {
-                e += qo;
-                if (e >= r) e -= r;
-                acc = acc + cmul(src[(long long)q * m], roots[e]);
-            }
-            B[(long long)seq * n + (j / Ns) * r * Ns + k + qo * Ns] = acc;
-        }
-        __syncthreads();
         cpx<T> *tmp = A;
         A = B;
         B = tmp;
@@ -88,37 +180,53 @@ __device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, cpx<T> *roots, int nseq, const 
     return A;
 }
 
-// Row r2c of any even Ny: one row per workgroup, L = Ny/2 complex points
-// z[n] = x[2n] + i x[2n+1], then the r2c split
+template <typename T> __device__ __forceinline__ cpx<T> *gen_lds()
+{
+    extern __shared__ __align__(16) unsigned char gen_smem[];
+    return reinterpret_cast<cpx<T> *>(gen_smem);
+}
+
+// Row r2c of any even Ny: nseq rows per workgroup, L = Ny/2 complex points
+// z[n] = x[2n] + i x[2n+1] each, then the r2c split
 //   X[k] = Fe + w_{Ny}^k Fo, X[L-k] = conj(Fe - w_{Ny}^k Fo),
 //   Fe = (Z[k] + conj Z[L-k]) / 2, Fo = (Z[k] - conj Z[L-k]) / (2i).
-// tabL: w_L^m (m < L); tab2: w_{Ny}^m (m < Ny).  blockIdx.x = row, .y = grid.
+// tabL: w_L^m (m < L); tab2: w_{Ny}^m (m < Ny).  blockIdx.x = row group
+// (XCD order), .y = grid.  Dynamic LDS: 2 * nseq * L complex.
 template <typename T>
 __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan g, const cpx<double> *tabL,
-                                                            const cpx<double> *tab2)
+                                                            const cpx<double> *tab2, int nseq)
 {
-    constexpr int CAP = gen_max_points<T>();
-    __shared__ cpx<T> buf[2 * CAP];
-    __shared__ cpx<T> roots[GEN_MAX_RADIX];
+    cpx<T> *buf = gen_lds<T>();
     const int t = threadIdx.x, L = g.n;
     const long long b = blockIdx.y;
-    const int row = blockIdx.x;
-    const T *in = reinterpret_cast<const T *>(a.in) + b * a.dist_in + (long long)row * a.ld_in;
-    T *out = reinterpret_cast<T *>(a.out) + b * a.dist_out + (long long)row * a.ld_out;
-    for (int e = t; e < L; e += GEN_NT) buf[e] = stor<T>::ld(in + 2 * e);
-    __syncthreads();
-    const cpx<T> *Z = gen_fft<T>(buf, buf + CAP, roots, 1, g, tabL);
-    for (int k = t; 2 * k <= L; k += GEN_NT)
+    const int row0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * nseq;
+    const int nrow = min(nseq, a.nrows - row0);
+    const T *in = reinterpret_cast<const T *>(a.in) + b * a.dist_in + (long long)row0 * a.ld_in;
+    T *out = reinterpret_cast<T *>(a.out) + b * a.dist_out + (long long)row0 * a.ld_out;
+    const float inv_l = 1.0f / (float)L;
+    for (int i = t; i < nseq * L; i += GEN_NT)
     {
+        const int s = gdiv(i, L, inv_l), e = i - s * L;
+        buf[i] = s < nrow ? stor<T>::ld(in + s * a.ld_in + 2 * e) : cpx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    const cpx<T> *Zs = gen_fft<T>(buf, buf + nseq * L, nseq, g, tabL);
+    const int half = L / 2 + 1;   // k = 0 .. floor(L/2)
+    const float inv_h = 1.0f / (float)half;
+    for (int i = t; i < nrow * half; i += GEN_NT)
+    {
+        const int s = gdiv(i, half, inv_h), k = i - s * half;
+        const cpx<T> *Z = Zs + s * L;
+        T *o = out + s * a.ld_out;
         if (k == 0)
         {
             const cpx<T> z0 = Z[0];
-            stor<T>::st(out, cpx<T>{z0.x + z0.y, T(0)});
-            stor<T>::st(out + 2 * L, cpx<T>{z0.x - z0.y, T(0)});
+            stor<T>::st(o, cpx<T>{z0.x + z0.y, T(0)});
+            stor<T>::st(o + 2 * L, cpx<T>{z0.x - z0.y, T(0)});
         }
         else if (2 * k == L)
         {
-            stor<T>::st(out + 2 * k, conjc(Z[k]));   // pairs with itself
+            stor<T>::st(o + 2 * k, conjc(Z[k]));   // pairs with itself
         }
         else
         {
@@ -126,10 +234,9 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
             const cpx<T> Fe = scal(A + Bc, T(0.5));
             const cpx<T> D = scal(A - Bc, T(0.5));
             const cpx<T> Fo = {D.y, -D.x};
-            const cpx<double> w64 = tab2[k];
-            const cpx<T> tt = cmul(cpx<T>{(T)w64.x, (T)w64.y}, Fo);
-            stor<T>::st(out + 2 * k, Fe + tt);
-            stor<T>::st(out + 2 * (L - k), conjc(Fe - tt));
+            const cpx<T> tt = cmul(tabv<T>(tab2, k), Fo);
+            stor<T>::st(o + 2 * k, Fe + tt);
+            stor<T>::st(o + 2 * (L - k), conjc(Fe - tt));
         }
     }
 }
@@ -137,79 +244,87 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
 // Column c2c of any Nx: a workgroup takes W adjacent columns over all Nx rows
 // (W * Nx <= gen_max_points), from a.in to a.out (the same spectrum for the
 // forward transform, in place).  CONJ: the inverse transform, computed as
-// conj(DFT(conj .)).  blockIdx.x = strip, .y = grid.  tab: w_Nx^m (m < Nx).
+// conj(DFT(conj .)).  blockIdx.x = strip (XCD order: neighbouring strips,
+// which share 128-byte lines, run behind one L2), .y = grid.
+// tab: w_Nx^m (m < Nx).  Dynamic LDS: 2 * W * Nx complex.
 template <typename T, bool CONJ>
 __global__ void __launch_bounds__(GEN_NT) gen_col_kernel(ColArgs a, GenPlan g, const cpx<double> *tab, int W)
 {
-    constexpr int CAP = gen_max_points<T>();
-    __shared__ cpx<T> buf[2 * CAP];
-    __shared__ cpx<T> roots[GEN_MAX_RADIX];
+    cpx<T> *buf = gen_lds<T>();
     const int t = threadIdx.x, n = g.n;
     const long long b = blockIdx.y;
-    const int col0 = blockIdx.x * W;
+    const int col0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * W;
     const int nvalid = min(W, a.ncols - col0);
     const T *src = reinterpret_cast<const T *>(a.in) + 2 * b * a.dist;
     T *dst = reinterpret_cast<T *>(a.out) + 2 * b * a.dist;
-    // load row segments: element (e, w) -> buf[w * n + e]
+    const float inv_w = 1.0f / (float)W;
+    // row segments: element (e, w) -> buf[w * n + e]
     for (int i = t; i < n * W; i += GEN_NT)
     {
-        const int e = i / W, w = i - e * W;
+        const int e = gdiv(i, W, inv_w), w = i - e * W;
         cpx<T> z = {T(0), T(0)};
         if (w < nvalid) z = stor<T>::ld(src + 2 * ((long long)e * a.pitch + col0 + w));
         buf[w * n + e] = CONJ ? conjc(z) : z;
     }
     __syncthreads();
-    const cpx<T> *X = gen_fft<T>(buf, buf + CAP, roots, W, g, tab);
+    const cpx<T> *X = gen_fft<T>(buf, buf + n * W, W, g, tab);
     for (int i = t; i < n * W; i += GEN_NT)
     {
-        const int e = i / W, w = i - e * W;
+        const int e = gdiv(i, W, inv_w), w = i - e * W;
         const cpx<T> z = X[w * n + e];
         if (w < nvalid) stor<T>::st(dst + 2 * ((long long)e * a.pitch + col0 + w), CONJ ? conjc(z) : z);
     }
 }
 
 // Row c2r of any even Ny (the row half of the adjoint when HALVE, the
-// unnormalised inverse c2r otherwise): half-spectrum row G[0..L] in, Ny reals out,
+// unnormalised inverse c2r otherwise): half-spectrum row G[0..L] in, Ny reals
+// out, nseq rows per workgroup (dynamic LDS: 2 * nseq * L complex),
 //   H = G on 0<k<L (halved when HALVE), H[0] = Re G[0], H[L] = Re G[L],
 //   Z[k] = (H[k] + conj H[L-k]) + i (H[k] - conj H[L-k]) conj(w_{Ny}^k),
 //   out[2n] + i out[2n+1] = IDFT_L(Z) = conj(DFT(conj Z)).
 template <typename T, bool HALVE>
 __global__ void __launch_bounds__(GEN_NT) gen_row_c2r_kernel(RowArgs a, GenPlan g, const cpx<double> *tabL,
-                                                            const cpx<double> *tab2)
+                                                            const cpx<double> *tab2, int nseq)
 {
-    constexpr int CAP = gen_max_points<T>();
-    __shared__ cpx<T> buf[2 * CAP];
-    __shared__ cpx<T> roots[GEN_MAX_RADIX];
+    cpx<T> *buf = gen_lds<T>();
     const int t = threadIdx.x, L = g.n;
     const long long b = blockIdx.y;
-    const int row = blockIdx.x;
-    const T *in = reinterpret_cast<const T *>(a.in) + b * a.dist_in + (long long)row * a.ld_in;
-    T *out = reinterpret_cast<T *>(a.out) + b * a.dist_out + (long long)row * a.ld_out;
+    const int row0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * nseq;
+    const int nrow = min(nseq, a.nrows - row0);
+    const T *in = reinterpret_cast<const T *>(a.in) + b * a.dist_in + (long long)row0 * a.ld_in;
+    T *out = reinterpret_cast<T *>(a.out) + b * a.dist_out + (long long)row0 * a.ld_out;
     const T hs = HALVE ? T(0.5) : T(1);
-    for (int k = t; k < L; k += GEN_NT)
+    const float inv_l = 1.0f / (float)L;
+    for (int i = t; i < nseq * L; i += GEN_NT)
     {
-        cpx<T> z;
-        if (k == 0)
+        const int s = gdiv(i, L, inv_l), k = i - s * L;
+        cpx<T> z = {T(0), T(0)};
+        if (s < nrow)
         {
-            const T h0 = stor<T>::ld(in).x, hl = stor<T>::ld(in + 2 * L).x;
-            z = cpx<T>{h0 + hl, h0 - hl};
+            const T *g_in = in + s * a.ld_in;
+            if (k == 0)
+            {
+                const T h0 = stor<T>::ld(g_in).x, hl = stor<T>::ld(g_in + 2 * L).x;
+                z = cpx<T>{h0 + hl, h0 - hl};
+            }
+            else
+            {
+                const cpx<T> hk = scal(stor<T>::ld(g_in + 2 * k), hs), hl = scal(stor<T>::ld(g_in + 2 * (L - k)), hs);
+                const cpx<T> w = tabv<T>(tab2, k);
+                const cpx<T> wc = {w.x, -w.y};
+                const cpx<T> s0 = hk + conjc(hl), d0 = cmul(hk - conjc(hl), wc);
+                z = cpx<T>{s0.x - d0.y, s0.y + d0.x};
+            }
         }
-        else
-        {
-            const cpx<T> hk = scal(stor<T>::ld(in + 2 * k), hs), hl = scal(stor<T>::ld(in + 2 * (L - k)), hs);
-            const cpx<double> w64 = tab2[k];
-            const cpx<T> wc = {(T)w64.x, -(T)w64.y};
-            const cpx<T> s0 = hk + conjc(hl), d0 = cmul(hk - conjc(hl), wc);
-            z = cpx<T>{s0.x - d0.y, s0.y + d0.x};
-        }
-        buf[k] = conjc(z);
+        buf[i] = conjc(z);
     }
     __syncthreads();
-    const cpx<T> *U = gen_fft<T>(buf, buf + CAP, roots, 1, g, tabL);
-    for (int e = t; e < L; e += GEN_NT)
+    const cpx<T> *U = gen_fft<T>(buf, buf + nseq * L, nseq, g, tabL);
+    for (int i = t; i < nrow * L; i += GEN_NT)
     {
-        const cpx<T> u = U[e];
-        stor<T>::st(out + 2 * e, cpx<T>{u.x, -u.y});
+        const int s = gdiv(i, L, inv_l), e = i - s * L;
+        const cpx<T> u = U[i];
+        stor<T>::st(out + s * a.ld_out + 2 * e, cpx<T>{u.x, -u.y});
     }
 }
 
