@@ -129,7 +129,8 @@ static int run_all()
         hpxfft::shared::loop f;
         try
         {
-            f.initialize(hpxfft::shared::vector_2d(8, 136, 1.0), "estimate");   // Ny = 134: Ny/2 = 67 is prime > 61
+            // Ny = 8198: Ny/2 = 4099 is prime, and its Bluestein FFT (8192 points) exceeds FP64's 4096
+            f.initialize(hpxfft::shared::vector_2d(8, 8200, 1.0), "estimate");
         }
         catch (const std::invalid_argument &)
         {

@@ -290,11 +290,116 @@ bool gen_factor(size_t n, hpxfft_dev::GenPlan &g)
     return true;
 }
 
+// a general plan for length n: Stockham stages when its prime factors are
+// at most GEN_MAX_RADIX and it fits `cap` points, else Bluestein's algorithm
+// through the power-of-two FFT of length >= 2n - 1 when that fits
+bool gen_setup(size_t n, size_t cap, hpxfft_dev::GenPlan &g)
+{
+    if (n <= cap && gen_factor(n, g))
+    {
+        g.len = (int)n;
+        g.off_c = g.off_b = 0;
+        return true;
+    }
+    size_t M = 1;
+    while (M < 2 * n - 1) M *= 2;
+    if (M > cap || !gen_factor(M, g)) return false;
+    g.len = (int)n;
+    g.off_c = (int)M;
+    g.off_b = (int)(M + n);
+    return true;
+}
+
+// table entries of a general plan: w_n^m (m < n), and for Bluestein the chirp
+// (len) and the kernel spectrum (n)
+size_t gen_table_size(const hpxfft_dev::GenPlan &g) { return g.len == g.n ? (size_t)g.n : (size_t)(2 * g.n + g.len); }
+
+// fill a general plan's table at h[2*off ...] (complex FP64 pairs), from long
+// double values: w_n^m exact (octant-reduced), the chirp c[j] = w_{2len}^{j^2
+// mod 2len} exact, and Bh = DFT_n(b) / n with b[m] = conj c[m], b[n-m] = conj
+// c[m] (0 < m < len), by a long-double radix-2 FFT
+void gen_table(const hpxfft_dev::GenPlan &g, std::vector<double> &h, size_t off)
+{
+    const uint64_t n = (uint64_t)g.n;
+    for (uint64_t m = 0; m < n; ++m)
+    {
+        long double c, sn;
+        twiddle_ld(m, n, c, sn);
+        h[2 * (off + m)] = (double)c;
+        h[2 * (off + m) + 1] = -(double)sn;
+    }
+    if (g.len == g.n) return;
+    const uint64_t len = (uint64_t)g.len;
+    std::vector<long double> cr(len), ci(len);
+    for (uint64_t j = 0; j < len; ++j)
+    {
+        long double c, sn;
+        twiddle_ld((j * j) % (2 * len), 2 * len, c, sn);
+        cr[j] = c;
+        ci[j] = -sn;
+        h[2 * (off + n + j)] = (double)c;
+        h[2 * (off + n + j) + 1] = -(double)sn;
+    }
+    std::vector<long double> br(n, 0.0L), bi(n, 0.0L);
+    for (uint64_t m = 0; m < len; ++m)
+    {
+        br[m] = cr[m];
+        bi[m] = -ci[m];
+        if (m > 0)
+        {
+            br[n - m] = cr[m];
+            bi[n - m] = -ci[m];
+        }
+    }
+    // iterative radix-2 DIT FFT (n a power of two), forward sign
+    for (uint64_t i = 1, j = 0; i < n; ++i)
+    {
+        uint64_t bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j)
+        {
+            std::swap(br[i], br[j]);
+            std::swap(bi[i], bi[j]);
+        }
+    }
+    for (uint64_t half = 1; half < n; half *= 2)
+    {
+        for (uint64_t k = 0; k < half; ++k)
+        {
+            long double c, sn;
+            twiddle_ld(k, 2 * half, c, sn);
+            const long double wr = c, wi = -sn;
+            for (uint64_t s0 = 0; s0 < n; s0 += 2 * half)
+            {
+                const uint64_t a = s0 + k, b = a + half;
+                const long double tr = br[b] * wr - bi[b] * wi, ti = br[b] * wi + bi[b] * wr;
+                br[b] = br[a] - tr;
+                bi[b] = bi[a] - ti;
+                br[a] += tr;
+                bi[a] += ti;
+            }
+        }
+    }
+    for (uint64_t m = 0; m < n; ++m)
+    {
+        h[2 * (off + n + len + m)] = (double)(br[m] / (long double)n);
+        h[2 * (off + n + len + m) + 1] = (double)(bi[m] / (long double)n);
+    }
+}
+
 std::string radices(const hpxfft_dev::GenPlan &g)
 {
     std::string r;
     for (int i = 0; i < g.nstages; ++i) r += (i ? "x" : "") + std::to_string(g.radix[i]);
     return r.empty() ? "none" : r;
+}
+
+std::string gen_method(const hpxfft_dev::GenPlan &g)
+{
+    if (g.len == g.n) return "mixed-radix Stockham in LDS, radices " + radices(g);
+    return "Bluestein (chirp convolution through a " + std::to_string(g.n) + "-point FFT in LDS, radices " +
+           radices(g) + ")";
 }
 
 std::string describe(const hpxfft_plan_s *p)
@@ -324,9 +429,8 @@ std::string describe(const hpxfft_plan_s *p)
     os << "hpxfft-hip plan (gfx950): Nx=" << p->nx << " Ny=" << p->ny << " batch=" << p->batch << " precision "
        << prec_name(p->prec) << " effort " << p->effort << "\n";
     if (p->gen_row)
-        os << "  row r2c kernel (general length): " << p->ny / 2 << "-point mixed-radix Stockham in LDS, radices "
-           << radices(p->grow) << ", " << p->gen_rows << " row(s)/workgroup, " << hpxfft_dev::GEN_NT
-           << " threads + r2c split\n";
+        os << "  row r2c kernel (general length): " << p->ny / 2 << "-point " << gen_method(p->grow) << ", "
+           << p->gen_rows << " row(s)/workgroup, " << hpxfft_dev::GEN_NT << " threads + r2c split\n";
     else
     {
         const geom rg = row_geom(p->prec, p->lgL);
@@ -335,8 +439,8 @@ std::string describe(const hpxfft_plan_s *p)
            << (rg.E < 16 ? rg.E : 16) << (rg.split ? ", LDS exchange in re/im halves" : "") << "\n";
     }
     if (p->gen_col)
-        os << "  column c2c kernel (general length): " << p->nx << "-point mixed-radix Stockham in LDS, radices "
-           << radices(p->gcol) << ", " << p->gen_w << " adjacent columns/workgroup\n";
+        os << "  column c2c kernel (general length): " << p->nx << "-point " << gen_method(p->gcol) << ", "
+           << p->gen_w << " adjacent columns/workgroup\n";
     else if (!p->four_step)
     {
         const geom cg = col_geom(p->prec, p->lgx);
@@ -407,7 +511,7 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         a.out = spec_out;
         const dim3 grid((unsigned)((p->ncols + p->gen_w - 1) / p->gen_w), (unsigned)p->batch);
         const dim3 block(hpxfft_dev::GEN_NT);
-        const size_t lds = gen_lds_bytes(p, (size_t)p->gen_w * p->nx);
+        const size_t lds = gen_lds_bytes(p, (size_t)p->gen_w * p->gcol.n);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab) + p->gtab_N;
         if (p->prec == P_F64)
         {
@@ -497,7 +601,7 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     {
         const int ns = p->gen_rows;
         const dim3 grid((unsigned)((p->nx + ns - 1) / ns), (unsigned)p->batch);
-        const size_t lds = gen_lds_bytes(p, (size_t)ns * (p->ny / 2));
+        const size_t lds = gen_lds_bytes(p, (size_t)ns * p->grow.n);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
         if (p->prec == P_F64)
             hipLaunchKernelGGL(hpxfft_dev::gen_row_r2c_kernel<double>, grid, dim3(hpxfft_dev::GEN_NT), lds, p->stream,
@@ -592,28 +696,28 @@ int plan_common(hpxfft_plan_s *p)
                                                      dims + ")");
         const size_t cap = p->prec == P_F64 ? hpxfft_dev::gen_max_points<double>()
                                             : hpxfft_dev::gen_max_points<float>();
-        if (p->gen_row && (p->ny / 2 > cap || !gen_factor(p->ny / 2, p->grow)))
-            return fail(HPXFFT_UNSUPPORTED_SIZE, "row length Ny/2 = " + std::to_string(p->ny / 2) +
-                                                     " needs prime factors <= " +
-                                                     std::to_string(hpxfft_dev::GEN_MAX_RADIX) + " and at most " +
-                                                     std::to_string(cap) + " points (got " + dims + ")");
-        if (p->gen_col && (p->nx > cap || !gen_factor(p->nx, p->gcol)))
-            return fail(HPXFFT_UNSUPPORTED_SIZE, "column length Nx = " + std::to_string(p->nx) +
-                                                     " needs prime factors <= " +
-                                                     std::to_string(hpxfft_dev::GEN_MAX_RADIX) + " and at most " +
-                                                     std::to_string(cap) + " points (got " + dims + ")");
+        auto why = [&](const char *what, size_t n) {
+            return std::string(what) + " = " + std::to_string(n) + " needs at most " + std::to_string(cap) +
+                   " points with prime factors <= " + std::to_string(hpxfft_dev::GEN_MAX_RADIX) +
+                   ", or a Bluestein FFT (the power of two >= 2n-1) of at most " + std::to_string(cap) +
+                   " points, for this precision (got " + dims + ")";
+        };
+        if (p->gen_row && !gen_setup(p->ny / 2, cap, p->grow))
+            return fail(HPXFFT_UNSUPPORTED_SIZE, why("row length Ny/2", p->ny / 2));
+        if (p->gen_col && !gen_setup(p->nx, cap, p->gcol))
+            return fail(HPXFFT_UNSUPPORTED_SIZE, why("column length Nx", p->nx));
         if (p->gen_row)
         {
             // short rows share a workgroup: about 2048 points per workgroup
-            const size_t L = p->ny / 2;
-            p->gen_rows = (int)std::max<size_t>(1, std::min(cap / L, std::min<size_t>(p->nx, 2048 / L)));
+            const size_t ST = (size_t)p->grow.n;
+            p->gen_rows = (int)std::max<size_t>(1, std::min(cap / ST, std::min<size_t>(p->nx, 2048 / ST)));
         }
         if (p->gen_col)
         {
             // one 128-byte row segment per workgroup where the LDS allows
             const size_t wmax = 128 / (2 * storage_bytes(p->prec));
             size_t w = 1;
-            while (w * 2 <= wmax && w * 2 * p->nx <= cap) w *= 2;
+            while (w * 2 <= wmax && w * 2 * (size_t)p->gcol.n <= cap) w *= 2;
             p->gen_w = (int)w;
         }
     }
@@ -636,23 +740,24 @@ int plan_common(hpxfft_plan_s *p)
     if (p->gen_row || p->gen_col)
     {
         // exact FP64 tables (the long-double evaluation of the power-of-two tables)
-        const size_t L = p->gen_row ? p->ny / 2 : 0, N2 = p->gen_row ? p->ny : 0, NX = p->gen_col ? p->nx : 0;
+        const size_t L = p->gen_row ? gen_table_size(p->grow) : 0, N2 = p->gen_row ? p->ny : 0;
+        const size_t NX = p->gen_col ? gen_table_size(p->gcol) : 0;
         p->gtab_L = 0;
         p->gtab_2 = L;
         p->gtab_N = L + N2;
         std::vector<double> h(2 * (L + N2 + NX));
-        auto fill = [&](size_t off, size_t n) {
-            for (size_t m = 0; m < n; ++m)
+        if (p->gen_row)
+        {
+            gen_table(p->grow, h, p->gtab_L);
+            for (size_t m = 0; m < N2; ++m)   // w_Ny^m for the r2c split
             {
                 long double c, sn;
-                twiddle_ld(m, n, c, sn);
-                h[2 * (off + m)] = (double)c;
-                h[2 * (off + m) + 1] = -(double)sn;
+                twiddle_ld(m, N2, c, sn);
+                h[2 * (p->gtab_2 + m)] = (double)c;
+                h[2 * (p->gtab_2 + m) + 1] = -(double)sn;
             }
-        };
-        fill(p->gtab_L, L);
-        fill(p->gtab_2, N2);
-        fill(p->gtab_N, NX);
+        }
+        if (p->gen_col) gen_table(p->gcol, h, p->gtab_N);
         HIPCHK(hipMalloc(&p->gen_tab, sizeof(double) * h.size()));
         HIPCHK(hipMemcpy(p->gen_tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
     }
@@ -893,7 +998,7 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     {
         const int ns = plan->gen_rows;
         const dim3 grid((unsigned)((plan->nx + ns - 1) / ns), (unsigned)plan->batch), block(hpxfft_dev::GEN_NT);
-        const size_t lds = gen_lds_bytes(plan, (size_t)ns * (plan->ny / 2));
+        const size_t lds = gen_lds_bytes(plan, (size_t)ns * plan->grow.n);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(plan->gen_tab);
         const auto *tL = tab + plan->gtab_L, *t2 = tab + plan->gtab_2;
         if (plan->prec == P_F64)

@@ -2,7 +2,8 @@
 // (FP64 / FP32).  The reference accepts any even Ny and any Nx through FFTW
 // (core/src/util/adapter_fftw.cpp:9,28-29); these kernels cover the lengths
 // whose prime factors are at most GEN_MAX_RADIX and whose data fit one
-// workgroup's LDS buffer, with the power-of-two kernels kept for
+// workgroup's LDS buffer, and any other length whose Bluestein FFT (a power
+// of two >= 2 len - 1) fits it, with the power-of-two kernels kept for
 // power-of-two dimensions.
 //
 // One workgroup transforms NSEQ sequences of length n held in LDS (sequence
@@ -29,9 +30,11 @@ template <typename T> constexpr int gen_max_points() { return GEN_LDS_BYTES / (2
 
 struct GenPlan
 {
-    int n;                          // transform length
+    int n;                          // FFT length run in LDS (the buffer stride of a sequence)
     int nstages;
     int radix[GEN_MAX_STAGES];      // stage radices, product n
+    int len;                        // DFT length: n, or the length Bluestein's algorithm maps to n
+    int off_c, off_b;               // Bluestein: chirp and kernel-spectrum offsets in the table
 };
 
 // x / d for 0 <= x < 2^20 and d >= 1, from inv = 1/d in FP32: the product is
@@ -157,7 +160,7 @@ __device__ void gen_stage_any(cpx<T> *A, cpx<T> *B, int nseq, int n, int r, int 
 // all stages of the plan on nseq sequences in A (B: the second buffer);
 // returns the buffer holding the result
 template <typename T>
-__device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
+__device__ cpx<T> *gen_stages(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
 {
     int Ns = 1;
     for (int s = 0; s < g.nstages; ++s)
@@ -180,6 +183,44 @@ __device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, con
     return A;
 }
 
+// The DFT of length g.len of nseq sequences held at A[s*g.n + e], e < g.len.
+// Direct plans (g.len == g.n) run the Stockham stages.  Otherwise Bluestein's
+// algorithm: with the chirp c[j] = w_{2 len}^{j^2} (tab + off_c),
+//   X[k] = c[k] sum_j (x[j] c[j]) conj(c[k-j]),
+// a circular convolution of length n >= 2 len - 1, computed as
+// IDFT(DFT(a) * Bh) with Bh = DFT(b) / n (tab + off_b, b[m] = conj c[|m|]);
+// the inverse DFT is conj(DFT(conj .)).  Returns the buffer with X at
+// [s*g.n + k], k < g.len.
+template <typename T>
+__device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
+{
+    if (g.len == g.n) return gen_stages<T>(A, B, nseq, g, tab);
+    const int t = threadIdx.x, n = g.n, len = g.len, total = nseq * n;
+    const cpx<double> *chirp = tab + g.off_c, *bh = tab + g.off_b;
+    const float inv_n = 1.0f / (float)n;
+    for (int i = t; i < total; i += GEN_NT)
+    {
+        const int e = i - gdiv(i, n, inv_n) * n;
+        A[i] = e < len ? cmul(A[i], tabv<T>(chirp, e)) : cpx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    cpx<T> *F = gen_stages<T>(A, B, nseq, g, tab);
+    for (int i = t; i < total; i += GEN_NT)
+    {
+        const int e = i - gdiv(i, n, inv_n) * n;
+        F[i] = conjc(cmul(F[i], tabv<T>(bh, e)));
+    }
+    __syncthreads();
+    cpx<T> *G = gen_stages<T>(F, F == A ? B : A, nseq, g, tab);
+    for (int i = t; i < total; i += GEN_NT)
+    {
+        const int e = i - gdiv(i, n, inv_n) * n;
+        if (e < len) G[i] = cmul(conjc(G[i]), tabv<T>(chirp, e));
+    }
+    __syncthreads();
+    return G;
+}
+
 template <typename T> __device__ __forceinline__ cpx<T> *gen_lds()
 {
     extern __shared__ __align__(16) unsigned char gen_smem[];
@@ -190,14 +231,14 @@ template <typename T> __device__ __forceinline__ cpx<T> *gen_lds()
 // z[n] = x[2n] + i x[2n+1] each, then the r2c split
 //   X[k] = Fe + w_{Ny}^k Fo, X[L-k] = conj(Fe - w_{Ny}^k Fo),
 //   Fe = (Z[k] + conj Z[L-k]) / 2, Fo = (Z[k] - conj Z[L-k]) / (2i).
-// tabL: w_L^m (m < L); tab2: w_{Ny}^m (m < Ny).  blockIdx.x = row group
-// (XCD order), .y = grid.  Dynamic LDS: 2 * nseq * L complex.
+// tabL: the row plan's table; tab2: w_{Ny}^m (m < Ny).  blockIdx.x = row group
+// (XCD order), .y = grid.  Dynamic LDS: 2 * nseq * g.n complex.
 template <typename T>
 __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan g, const cpx<double> *tabL,
                                                             const cpx<double> *tab2, int nseq)
 {
     cpx<T> *buf = gen_lds<T>();
-    const int t = threadIdx.x, L = g.n;
+    const int t = threadIdx.x, L = g.len, ST = g.n;
     const long long b = blockIdx.y;
     const int row0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * nseq;
     const int nrow = min(nseq, a.nrows - row0);
@@ -207,16 +248,16 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
     for (int i = t; i < nseq * L; i += GEN_NT)
     {
         const int s = gdiv(i, L, inv_l), e = i - s * L;
-        buf[i] = s < nrow ? stor<T>::ld(in + s * a.ld_in + 2 * e) : cpx<T>{T(0), T(0)};
+        buf[s * ST + e] = s < nrow ? stor<T>::ld(in + s * a.ld_in + 2 * e) : cpx<T>{T(0), T(0)};
     }
     __syncthreads();
-    const cpx<T> *Zs = gen_fft<T>(buf, buf + nseq * L, nseq, g, tabL);
+    const cpx<T> *Zs = gen_fft<T>(buf, buf + nseq * ST, nseq, g, tabL);
     const int half = L / 2 + 1;   // k = 0 .. floor(L/2)
     const float inv_h = 1.0f / (float)half;
     for (int i = t; i < nrow * half; i += GEN_NT)
     {
         const int s = gdiv(i, half, inv_h), k = i - s * half;
-        const cpx<T> *Z = Zs + s * L;
+        const cpx<T> *Z = Zs + s * ST;
         T *o = out + s * a.ld_out;
         if (k == 0)
         {
@@ -242,16 +283,16 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
 }
 
 // Column c2c of any Nx: a workgroup takes W adjacent columns over all Nx rows
-// (W * Nx <= gen_max_points), from a.in to a.out (the same spectrum for the
+// (W * g.n <= gen_max_points), from a.in to a.out (the same spectrum for the
 // forward transform, in place).  CONJ: the inverse transform, computed as
 // conj(DFT(conj .)).  blockIdx.x = strip (XCD order: neighbouring strips,
 // which share 128-byte lines, run behind one L2), .y = grid.
-// tab: w_Nx^m (m < Nx).  Dynamic LDS: 2 * W * Nx complex.
+// tab: the column plan's table.  Dynamic LDS: 2 * W * g.n complex.
 template <typename T, bool CONJ>
 __global__ void __launch_bounds__(GEN_NT) gen_col_kernel(ColArgs a, GenPlan g, const cpx<double> *tab, int W)
 {
     cpx<T> *buf = gen_lds<T>();
-    const int t = threadIdx.x, n = g.n;
+    const int t = threadIdx.x, n = g.len, ST = g.n;
     const long long b = blockIdx.y;
     const int col0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * W;
     const int nvalid = min(W, a.ncols - col0);
@@ -264,21 +305,21 @@ __global__ void __launch_bounds__(GEN_NT) gen_col_kernel(ColArgs a, GenPlan g, c
         const int e = gdiv(i, W, inv_w), w = i - e * W;
         cpx<T> z = {T(0), T(0)};
         if (w < nvalid) z = stor<T>::ld(src + 2 * ((long long)e * a.pitch + col0 + w));
-        buf[w * n + e] = CONJ ? conjc(z) : z;
+        buf[w * ST + e] = CONJ ? conjc(z) : z;
     }
     __syncthreads();
-    const cpx<T> *X = gen_fft<T>(buf, buf + n * W, W, g, tab);
+    const cpx<T> *X = gen_fft<T>(buf, buf + ST * W, W, g, tab);
     for (int i = t; i < n * W; i += GEN_NT)
     {
         const int e = gdiv(i, W, inv_w), w = i - e * W;
-        const cpx<T> z = X[w * n + e];
+        const cpx<T> z = X[w * ST + e];
         if (w < nvalid) stor<T>::st(dst + 2 * ((long long)e * a.pitch + col0 + w), CONJ ? conjc(z) : z);
     }
 }
 
 // Row c2r of any even Ny (the row half of the adjoint when HALVE, the
 // unnormalised inverse c2r otherwise): half-spectrum row G[0..L] in, Ny reals
-// out, nseq rows per workgroup (dynamic LDS: 2 * nseq * L complex),
+// out, nseq rows per workgroup (dynamic LDS: 2 * nseq * g.n complex),
 //   H = G on 0<k<L (halved when HALVE), H[0] = Re G[0], H[L] = Re G[L],
 //   Z[k] = (H[k] + conj H[L-k]) + i (H[k] - conj H[L-k]) conj(w_{Ny}^k),
 //   out[2n] + i out[2n+1] = IDFT_L(Z) = conj(DFT(conj Z)).
@@ -287,7 +328,7 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_c2r_kernel(RowArgs a, GenPlan 
                                                             const cpx<double> *tab2, int nseq)
 {
     cpx<T> *buf = gen_lds<T>();
-    const int t = threadIdx.x, L = g.n;
+    const int t = threadIdx.x, L = g.len, ST = g.n;
     const long long b = blockIdx.y;
     const int row0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * nseq;
     const int nrow = min(nseq, a.nrows - row0);
@@ -316,14 +357,14 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_c2r_kernel(RowArgs a, GenPlan 
                 z = cpx<T>{s0.x - d0.y, s0.y + d0.x};
             }
         }
-        buf[i] = conjc(z);
+        buf[s * ST + k] = conjc(z);
     }
     __syncthreads();
-    const cpx<T> *U = gen_fft<T>(buf, buf + nseq * L, nseq, g, tabL);
+    const cpx<T> *U = gen_fft<T>(buf, buf + nseq * ST, nseq, g, tabL);
     for (int i = t; i < nrow * L; i += GEN_NT)
     {
         const int s = gdiv(i, L, inv_l), e = i - s * L;
-        const cpx<T> u = U[i];
+        const cpx<T> u = U[s * ST + e];
         stor<T>::st(out + s * a.ld_out + 2 * e, cpx<T>{u.x, -u.y});
     }
 }

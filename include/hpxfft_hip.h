@@ -38,8 +38,10 @@ typedef enum
     HPXFFT_OK = 0,
     HPXFFT_INVALID_ARG = 1,      /* bad pointer, flag, layout, or call order */
     HPXFFT_UNSUPPORTED_SIZE = 2, /* size outside what the kernels cover: powers of two (Nx <= 2^16,
-                                    Ny <= 2^15), or for F64/F32 plans any Nx and even Ny
-                                    with prime factors <= 61 within 4096 (F64) / 8192 (F32) points */
+                                    Ny <= 2^15), or for F64/F32 plans any Nx and even Ny whose
+                                    lengths (Nx, Ny/2) have prime factors <= 61 within 4096 (F64) /
+                                    8192 (F32) points, or otherwise up to 2048 (F64) / 4096 (F32)
+                                    points by Bluestein's algorithm */
     HPXFFT_NO_DEVICE = 3,        /* no HIP device visible */
     HPXFFT_OUT_OF_MEMORY = 4,
     HPXFFT_HIP_ERROR = 5,

@@ -72,12 +72,24 @@ def plan_status(nx, ny, batch=1, ld_in=None, dist_in=None, ld_out=None, dist_out
 
 
 @pytest.mark.parametrize("nx,ny", [(16, 1), (16, 15), (0, 16), (16, 0), (1 << 17, 4), (4, 1 << 16),
-                                   (16, 134), (67, 16), (16, 2 * 8193), (6000, 8)])
+                                   (16, 2 * 4099), (4099, 16), (16, 2 * 8193), (6000, 8)])
 def test_unsupported_sizes(nx, ny):
-    """Odd Ny, zero sizes, powers of two beyond the kernels' range, and other
-    lengths with a prime factor above 61 or beyond one workgroup's LDS."""
+    """Odd Ny, zero sizes, powers of two beyond the kernels' range, and FP64
+    lengths beyond one workgroup's LDS: more than 4096 points, or a prime
+    factor above 61 whose Bluestein FFT (the power of two >= 2n-1) exceeds
+    4096 points (4099 is prime)."""
     assert plan_status(nx, ny) == _lib.UNSUPPORTED_SIZE
     assert hpxfft_amd.last_error()
+
+
+def test_fp32_bluestein_reaches_twice_as_far():
+    """FP32 holds 8192 points per buffer: Ny/2 = 4093 (prime) fits its
+    Bluestein FFT, which FP64 (4096 points) rejects."""
+    assert plan_status(16, 2 * 4093) == _lib.UNSUPPORTED_SIZE
+    st = plan_status(16, 2 * 4093, prec=1)
+    if st == _lib.OK:
+        pytest.skip("a HIP device is present")
+    assert st == _lib.NO_DEVICE, hpxfft_amd.last_error()
 
 
 @pytest.mark.parametrize("prec", [2, 3])
@@ -86,10 +98,12 @@ def test_general_lengths_need_f64_or_f32(prec):
     assert "F64 or F32" in hpxfft_amd.last_error()
 
 
-@pytest.mark.parametrize("nx,ny", [(8, 14), (3, 16), (1000, 64), (64, 1000), (4095, 2)])
+@pytest.mark.parametrize("nx,ny", [(8, 14), (3, 16), (1000, 64), (64, 1000), (4095, 2),
+                                   (16, 134), (67, 16), (2039, 2 * 1031)])
 def test_general_lengths_pass_validation(nx, ny):
-    """Lengths that are not powers of two are accepted (without a device the
-    plan then stops at the device check)."""
+    """Lengths that are not powers of two are accepted, prime factors above 61
+    through Bluestein's algorithm (without a device the plan then stops at the
+    device check)."""
     st = plan_status(nx, ny)
     if st == _lib.OK:
         pytest.skip("a HIP device is present")
@@ -157,8 +171,8 @@ def test_cpp_harness_argument_errors(tmp_path):
     r = subprocess.run([exe, "--plan=bogus", "--nx", "4", "--ny", "4"], capture_output=True, text=True, timeout=60,
                        cwd=tmp_path)
     assert r.returncode == 2 and "Invalid FFTW plan flag string" in r.stderr
-    r = subprocess.run([exe, "--nx", "8", "--ny", "134"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
-    assert r.returncode == 2 and "prime factors" in r.stderr       # Ny/2 = 67
+    r = subprocess.run([exe, "--nx", "8", "--ny", "8198"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    assert r.returncode == 2 and "prime factors" in r.stderr       # Ny/2 = 4099, prime
 
 
 @pytest.mark.parametrize("args,msg", [
@@ -186,7 +200,7 @@ def test_cpp_harness_bool_words(tmp_path, word):
     reference's size sweep passes --header=true.  The unsupported size makes
     the run stop after argument parsing, at plan validation (status 2)."""
     r = subprocess.run([os.path.join(BIN, "shared_loop_2d_gpu"), "--header=" + word, "--result", word, "--nx=8",
-                        "--ny=134"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+                        "--ny=8198"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
     assert r.returncode == 2 and "prime factors" in r.stderr and "usage error" not in r.stderr, r.stderr
 
 

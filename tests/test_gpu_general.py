@@ -1,7 +1,9 @@
 """Lengths that are not powers of two (the general mixed-radix kernels) on the GPU.
 
 The reference accepts any even Ny and any Nx through FFTW; the general
-kernels cover lengths with prime factors <= 61 that fit one workgroup's LDS.
+kernels cover lengths with prime factors <= 61 that fit one workgroup's LDS,
+and other lengths by Bluestein's algorithm (a power-of-two FFT of at least
+2n-1 points in LDS).
 Checks against the oracle (itself pinned to brute force and numpy for these
 lengths): FP64 <= 1e-12, FP32 <= 1e-5 relative L2, with general rows and/or
 general columns combined with every power-of-two kernel family; batches;
@@ -28,6 +30,11 @@ SIZES = [
     (3000, 8), (6, 8190),
     (4096, 30),     # general rows, four-step columns
     (10, 4096),     # general columns, long power-of-two rows
+    # prime factors above 61: Bluestein's algorithm
+    (8, 134), (67, 16), (67, 134), (127, 262), (1031, 16),
+    (4096, 134),    # Bluestein rows, four-step columns
+    (67, 4096),     # Bluestein columns, long power-of-two rows
+    (3, 2 * 2039),  # Bluestein rows through a 4096-point FFT (the FP64 limit)
 ]
 
 
@@ -63,6 +70,14 @@ def test_general_fp32_beyond_fp64_lds(gpu):
     assert oracle.rel_l2(got, oracle.r2c_2d(x, ny)) <= 1e-5
 
 
+def test_bluestein_fp32_beyond_fp64_lds(gpu):
+    """FP32 runs Bluestein FFTs of up to 8192 points: Ny/2 = 4093 (prime)."""
+    nx, ny = 4, 2 * 4093
+    x = oracle.fill(nx, ny, seed=12, dist="normal")
+    got = _run(x, nx, ny, "f32")[:, : 2 * (ny // 2 + 1)].copy().view(np.complex128)
+    assert oracle.rel_l2(got, oracle.r2c_2d(x, ny)) <= 1e-5
+
+
 def test_general_batched(gpu):
     nx, ny, batch = 30, 42, 3
     ld = 2 * (ny // 2 + 1)
@@ -73,7 +88,8 @@ def test_general_batched(gpu):
         assert oracle.rel_l2(g, oracle.r2c_2d(x, ny)) <= 1e-12, b
 
 
-BACK_SIZES = [(8, 14), (5, 6), (1, 6), (12, 100), (1000, 64), (64, 1000), (4096, 30), (10, 4096), (3, 2)]
+BACK_SIZES = [(8, 14), (5, 6), (1, 6), (12, 100), (1000, 64), (64, 1000), (4096, 30), (10, 4096), (3, 2),
+              (67, 134), (127, 262), (4096, 134)]
 
 
 @pytest.mark.parametrize("nx,ny", BACK_SIZES)
