@@ -14,8 +14,9 @@
 //     B[(j / Ns) r Ns + k + q Ns] = y_q
 // Radices 2, 3, 4, 5 and 7 run as register butterflies (one LDS pass per
 // stage); larger primes (up to 61) by a generic stage of two passes.
-// Twiddles w_n^m and the roots w_r^e = w_n^{e n/r} come from the plan's
-// exact FP64 table (rounded once for FP32).
+// Twiddles and the roots w_r^e = w_n^{e n/r} come from the plan's exact FP64
+// table (rounded once for FP32); a register butterfly loads w^k and forms
+// w^{qk} (q < r <= 7) as FP64 powers of it.
 #pragma once
 
 #include "fft_kernels.hpp"
@@ -102,23 +103,49 @@ __device__ void gen_stage_reg(const cpx<T> *A, cpx<T> *B, int nseq, int n, int N
 #pragma unroll
     for (int e = 1; e < R; ++e) w[e] = tabv<T>(tab, e * m);
     w[0] = cpx<T>{T(1), T(0)};
-    for (int b = t; b < nb; b += GEN_NT)
+    // two butterflies per iteration (independent loads in flight together);
+    // twiddles w^{qk} as powers of the one table value w^k, in FP64
+    for (int b0 = t; b0 < nb; b0 += 2 * GEN_NT)
     {
-        const int seq = gdiv(b, m, inv_m), j = b - seq * m;
-        const int jb = gdiv(j, Ns, inv_ns), k = j - jb * Ns;
-        const cpx<T> *src = A + seq * n + j;
-        cpx<T> x[R];
+        cpx<T> x[2][R];
+        cpx<double> w1[2];
+        int dst[2] = {-1, -1}, kk[2] = {0, 0};
 #pragma unroll
-        for (int q = 0; q < R; ++q) x[q] = src[q * m];
-        if (k > 0)
+        for (int u = 0; u < 2; ++u)
         {
+            const int b = b0 + u * GEN_NT;
+            if (b < nb)
+            {
+                const int seq = gdiv(b, m, inv_m), j = b - seq * m;
+                const int jb = gdiv(j, Ns, inv_ns), k = j - jb * Ns;
+                const cpx<T> *src = A + seq * n + j;
 #pragma unroll
-            for (int q = 1; q < R; ++q) x[q] = cmul(x[q], tabv<T>(tab, q * k * tstep));
+                for (int q = 0; q < R; ++q) x[u][q] = src[q * m];
+                w1[u] = tab[k * tstep];
+                kk[u] = k;
+                dst[u] = seq * n + jb * R * Ns + k;
+            }
         }
-        dft_small<T, R>(x, w);
-        cpx<T> *dst = B + seq * n + jb * R * Ns + k;
 #pragma unroll
-        for (int q = 0; q < R; ++q) dst[q * Ns] = x[q];
+        for (int u = 0; u < 2; ++u)
+        {
+            if (dst[u] < 0) continue;
+            if (kk[u] > 0)
+            {
+                cpx<double> wq = w1[u];
+                x[u][1] = cmul(x[u][1], cpx<T>{(T)wq.x, (T)wq.y});
+#pragma unroll
+                for (int q = 2; q < R; ++q)
+                {
+                    wq = cmul(wq, w1[u]);
+                    x[u][q] = cmul(x[u][q], cpx<T>{(T)wq.x, (T)wq.y});
+                }
+            }
+            dft_small<T, R>(x[u], w);
+            cpx<T> *dst_p = B + dst[u];
+#pragma unroll
+            for (int q = 0; q < R; ++q) dst_p[q * Ns] = x[u][q];
+        }
     }
     __syncthreads();
 }
