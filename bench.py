@@ -264,24 +264,25 @@ def bench_mixed(n: int = 1 << 15, reps: int = 3) -> dict:
     return out
 
 
-def bench_general(n: int = 3000, reps: int = 10) -> dict:
-    """A length that is not a power of two: 3000 x 3000 FP64 (radices 4, 2, 3, 5)
-    on the general mixed-radix kernels, 2-pass byte model as the headline."""
+def bench_general(nx: int = 3000, ny: int = 3000, reps: int = 10) -> dict:
+    """Lengths that are not powers of two, FP64, on the general kernels
+    (2-pass byte model): by default 3000 x 3000 (radices 4, 2, 3, 5); with
+    prime lengths such as 2039 x 4078 the Bluestein path."""
     import torch
     import hpxfft_amd
 
-    ld = 2 * (n // 2 + 1)
+    ld = 2 * (ny // 2 + 1)
     g = torch.Generator(device="cuda").manual_seed(6)
-    x = torch.empty((n, ld), dtype=torch.float64, device="cuda")
-    x[:, :n].uniform_(-1.0, 1.0, generator=g)
-    x[:, n:].zero_()
-    out = torch.empty((n, n // 2 + 1), dtype=torch.complex128, device="cuda")
-    p = hpxfft_amd.Plan(n, n, ld_in=ld, precision="f64")
+    x = torch.empty((nx, ld), dtype=torch.float64, device="cuda")
+    x[:, :ny].uniform_(-1.0, 1.0, generator=g)
+    x[:, ny:].zero_()
+    out = torch.empty((nx, ny // 2 + 1), dtype=torch.complex128, device="cuda")
+    p = hpxfft_amd.Plan(nx, ny, ld_in=ld, precision="f64")
     ms = _events_ms(lambda: p.exec(x.data_ptr(), out.data_ptr()), reps)
     desc = [ln.strip() for ln in p.describe().splitlines() if "general" in ln]
     p.close()
-    B = alg_bytes(n, n, 8)["total"]
-    return {"config": f"{n}x{n} FP64", "ms": round(ms, 4), "GBps": round(B / (ms * 1e-3) / 1e9, 1),
+    B = alg_bytes(nx, ny, 8)["total"]
+    return {"config": f"{nx}x{ny} FP64", "ms": round(ms, 4), "GBps": round(B / (ms * 1e-3) / 1e9, 1),
             "kernels": desc}
 
 
@@ -408,6 +409,8 @@ def main() -> int:
         progress("2^15 mixed precision done")
         torch.cuda.empty_cache()
         result["general_length"] = bench_general()
+        progress("general lengths: smooth done")
+        result["general_length_bluestein"] = bench_general(2039, 2 * 2039)
         progress("general length done")
         torch.cuda.empty_cache()
     if not args.no_cpu:
