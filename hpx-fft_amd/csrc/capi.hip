@@ -173,6 +173,17 @@ const char *prec_name(int prec)
 }   // namespace
 
 // ---------------------------------------------------------------------------
+// a dimension of length N = P * Q beyond one workgroup: phase A runs the
+// Q-point plan A over elements m1 + P*e, phase B the P-point plan B
+struct gen2_dim
+{
+    bool on = false;
+    int P = 0, Q = 0;
+    hpxfft_dev::GenPlan A{}, B{};
+    int WA = 1, WB = 1;                    // sequences per workgroup in each phase
+    size_t tabA = 0, tabB = 0, tabT = 0;   // table offsets (complex entries): plans A, B and w_N^m
+};
+
 struct hpxfft_plan_s
 {
     // problem
@@ -192,6 +203,9 @@ struct hpxfft_plan_s
     hpxfft_dev::GenPlan grow{}, gcol{};
     int gen_w = 1;                 // general column kernel: adjacent columns per workgroup
     int gen_rows = 1;              // general row kernels: rows per workgroup
+    gen2_dim g2r, g2c;             // two-level general rows (Ny/2 = P*Q) / columns (Nx = P*Q)
+    void *gtmp = nullptr;          // two-level rows: phase A output, Nx x Ny/2 per grid (compute type)
+    void *gscr = nullptr;          // two-level columns: phase A output, Nx x (Ny/2+1) per grid
     double *gen_tab = nullptr;     // exact FP64 tables w_n^m: rows L = Ny/2, split Ny, columns Nx
     size_t gtab_L = 0, gtab_2 = 0, gtab_N = 0;   // their offsets (complex entries)
     // device memory
@@ -254,7 +268,8 @@ int gen_allow_lds(int dev)
         reinterpret_cast<const void *>(&gen_col_kernel<double, false>), reinterpret_cast<const void *>(&gen_col_kernel<double, true>),
         reinterpret_cast<const void *>(&gen_col_kernel<float, false>), reinterpret_cast<const void *>(&gen_col_kernel<float, true>),
         reinterpret_cast<const void *>(&gen_row_c2r_kernel<double, false>), reinterpret_cast<const void *>(&gen_row_c2r_kernel<double, true>),
-        reinterpret_cast<const void *>(&gen_row_c2r_kernel<float, false>), reinterpret_cast<const void *>(&gen_row_c2r_kernel<float, true>)};
+        reinterpret_cast<const void *>(&gen_row_c2r_kernel<float, false>), reinterpret_cast<const void *>(&gen_row_c2r_kernel<float, true>),
+        reinterpret_cast<const void *>(&gen_strided_kernel<double>), reinterpret_cast<const void *>(&gen_strided_kernel<float>)};
     for (const void *f : fns) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, GEN_LDS_BYTES));
     done[dev] = true;
     return HPXFFT_OK;
@@ -308,6 +323,44 @@ bool gen_setup(size_t n, size_t cap, hpxfft_dev::GenPlan &g)
     g.off_c = (int)M;
     g.off_b = (int)(M + n);
     return true;
+}
+
+// two-level split N = P * Q (N <= 2^20): both sub-lengths within one
+// workgroup, direct plans preferred over Bluestein, then the most balanced
+bool gen2_setup(size_t N, size_t cap, gen2_dim &d)
+{
+    if (N > (size_t(1) << 20)) return false;
+    int best = -1;
+    double best_score = 0.0;
+    for (size_t P = 2; P <= cap && P <= N / 2; ++P)
+    {
+        if (N % P) continue;
+        const size_t Q = N / P;
+        if (Q < 2 || Q > cap) continue;
+        hpxfft_dev::GenPlan A, B;
+        if (!gen_setup(Q, cap, A) || !gen_setup(P, cap, B)) continue;
+        const double score = 10.0 * ((A.len != A.n) + (B.len != B.n)) + std::fabs(std::log2((double)P / (double)Q));
+        if (best < 0 || score < best_score)
+        {
+            best = (int)P;
+            best_score = score;
+            d.A = A;
+            d.B = B;
+        }
+    }
+    if (best < 0) return false;
+    d.on = true;
+    d.P = best;
+    d.Q = (int)(N / (size_t)best);
+    return true;
+}
+
+// sequences per workgroup: up to `wmax` (a power of two) within `cap` points
+int gen_width(const hpxfft_dev::GenPlan &g, size_t cap, size_t wmax)
+{
+    size_t w = 1;
+    while (w * 2 <= wmax && w * 2 * (size_t)g.n <= cap) w *= 2;
+    return (int)w;
 }
 
 // table entries of a general plan: w_n^m (m < n), and for Bluestein the chirp
@@ -428,7 +481,18 @@ std::string describe(const hpxfft_plan_s *p)
     }
     os << "hpxfft-hip plan (gfx950): Nx=" << p->nx << " Ny=" << p->ny << " batch=" << p->batch << " precision "
        << prec_name(p->prec) << " effort " << p->effort << "\n";
-    if (p->gen_row)
+    auto two_level = [&](const char *what, const gen2_dim &d) {
+        os << "  " << what << " (general length, two levels): " << (size_t)d.P * d.Q << " = P*Q = " << d.P << " * "
+           << d.Q << "\n    phase A: " << d.Q << "-point " << gen_method(d.A) << " over elements m1+P*e, twiddle w^(m1 k2), "
+           << d.WA << " sequence(s)/workgroup\n    phase B: " << d.P << "-point " << gen_method(d.B) << ", "
+           << d.WB << " sequence(s)/workgroup\n";
+    };
+    if (p->g2r.on)
+    {
+        two_level("row r2c", p->g2r);
+        os << "    then the r2c split in place (one thread per pair k, Ny/2-k)\n";
+    }
+    else if (p->gen_row)
         os << "  row r2c kernel (general length): " << p->ny / 2 << "-point " << gen_method(p->grow) << ", "
            << p->gen_rows << " row(s)/workgroup, " << hpxfft_dev::GEN_NT << " threads + r2c split\n";
     else
@@ -438,7 +502,8 @@ std::string describe(const hpxfft_plan_s *p)
            << rg.nseq << " row(s)/workgroup, " << rg.nt << " threads, " << rg.E << " elements/thread, radix "
            << (rg.E < 16 ? rg.E : 16) << (rg.split ? ", LDS exchange in re/im halves" : "") << "\n";
     }
-    if (p->gen_col)
+    if (p->g2c.on) two_level("column c2c", p->g2c);
+    else if (p->gen_col)
         os << "  column c2c kernel (general length): " << p->nx << "-point " << gen_method(p->gcol) << ", "
            << p->gen_w << " adjacent columns/workgroup\n";
     else if (!p->four_step)
@@ -489,6 +554,139 @@ int alloc_events(hpxfft_plan_s *p)
     return HPXFFT_OK;
 }
 
+// one phase of a two-level general dimension (gen_strided_kernel)
+int launch_strided(hpxfft_plan_s *p, const hpxfft_dev::GenStride &a, const hpxfft_dev::GenPlan &g, size_t tab_off,
+                   size_t twt_off, int W, unsigned gx, unsigned gy, unsigned gz)
+{
+    const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
+    const size_t lds = gen_lds_bytes(p, (size_t)W * g.n);
+    const dim3 grid(gx, gy, gz), block(hpxfft_dev::GEN_NT);
+    if (p->prec == P_F64)
+        hipLaunchKernelGGL(hpxfft_dev::gen_strided_kernel<double>, grid, block, lds, p->stream, a, g, tab + tab_off,
+                           tab + twt_off, tab + p->gtab_2, W);
+    else
+        hipLaunchKernelGGL(hpxfft_dev::gen_strided_kernel<float>, grid, block, lds, p->stream, a, g, tab + tab_off,
+                           tab + twt_off, tab + p->gtab_2, W);
+    HIPCHK(hipGetLastError());
+    return HPXFFT_OK;
+}
+
+unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// two-level column pass (Nx = P*Q) from spec_in to spec_out through the
+// plan's scratch [m1][k2][col]; conj: the inverse transform
+int run_columns_2level(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj)
+{
+    const gen2_dim &d = p->g2c;
+    const long long P = d.P, Q = d.Q, nc = (long long)p->ncols, ld = (long long)p->ld_out;
+    hpxfft_dev::GenStride a{};
+    a.in = spec_in;
+    a.in_seq = P * ld;
+    a.in_col = 1;
+    a.in_sub = ld;
+    a.in_bat = (long long)p->dist_out;
+    a.out = p->gscr;
+    a.out_seq = nc;
+    a.out_col = 1;
+    a.out_sub = Q * nc;
+    a.out_bat = (long long)p->nx * nc;
+    a.ncols = (int)nc;
+    a.tw_n = (int)p->nx;
+    a.conj_in = conj;
+    if (int rc = launch_strided(p, a, d.A, d.tabA, d.tabT, d.WA, cdiv(nc, d.WA), (unsigned)P, (unsigned)p->batch))
+        return rc;
+    hpxfft_dev::GenStride b{};
+    b.in = p->gscr;
+    b.in_seq = Q * nc;
+    b.in_col = 1;
+    b.in_sub = nc;
+    b.in_bat = (long long)p->nx * nc;
+    b.out = spec_out;
+    b.out_seq = Q * ld;
+    b.out_col = 1;
+    b.out_sub = ld;
+    b.out_bat = (long long)p->dist_out;
+    b.ncols = (int)nc;
+    b.conj_out = conj;
+    return launch_strided(p, b, d.B, d.tabB, d.tabT, d.WB, cdiv(nc, d.WB), (unsigned)Q, (unsigned)p->batch);
+}
+
+// two-level rows (Ny/2 = L = P*Q).  Forward: the real rows of `real` as
+// complex points -> plan scratch -> the spectrum rows of `spec`, then the
+// r2c split in place.  Backward (c2r, halve for the adjoint): the c2r
+// sequence of each half-spectrum row of `spec` -> scratch -> `real`.
+int run_rows_2level(hpxfft_plan_s *p, const void *real_in, void *real_out, const void *spec_in, void *spec_out,
+                    bool backward, bool halve)
+{
+    const gen2_dim &d = p->g2r;
+    const long long P = d.P, Q = d.Q, L = P * Q;
+    hpxfft_dev::GenStride a{};
+    a.in_seq = P;
+    a.in_col = 1;
+    if (!backward)
+    {
+        a.in = real_in;
+        a.in_sub = (long long)p->ld_in / 2;
+        a.in_bat = (long long)p->dist_in / 2;
+    }
+    else
+    {
+        a.in = spec_in;
+        a.in_sub = (long long)p->ld_out;
+        a.in_bat = (long long)p->dist_out;
+        a.pre = halve ? 2 : 1;
+        a.prel = (int)L;
+        a.conj_in = 1;
+    }
+    a.out = p->gtmp;
+    a.out_seq = 1;
+    a.out_col = Q;
+    a.out_sub = L;
+    a.out_bat = (long long)p->nx * L;
+    a.ncols = (int)P;
+    a.tw_n = (int)L;
+    a.tw_col = 1;
+    if (int rc = launch_strided(p, a, d.A, d.tabA, d.tabT, d.WA, cdiv(P, d.WA), (unsigned)p->nx, (unsigned)p->batch))
+        return rc;
+    hpxfft_dev::GenStride b{};
+    b.in = p->gtmp;
+    b.in_seq = Q;
+    b.in_col = 1;
+    b.in_sub = L;
+    b.in_bat = (long long)p->nx * L;
+    b.out_seq = Q;
+    b.out_col = 1;
+    b.ncols = (int)Q;
+    if (!backward)
+    {
+        b.out = spec_out;
+        b.out_sub = (long long)p->ld_out;
+        b.out_bat = (long long)p->dist_out;
+    }
+    else
+    {
+        b.out = real_out;
+        b.out_sub = (long long)p->ld_in / 2;
+        b.out_bat = (long long)p->dist_in / 2;
+        b.conj_out = 1;
+    }
+    if (int rc = launch_strided(p, b, d.B, d.tabB, d.tabT, d.WB, cdiv(Q, d.WB), (unsigned)p->nx, (unsigned)p->batch))
+        return rc;
+    if (!backward)
+    {
+        const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
+        const dim3 grid(cdiv((size_t)L / 2 + 1, 256), (unsigned)p->nx, (unsigned)p->batch);
+        if (p->prec == P_F64)
+            hipLaunchKernelGGL(hpxfft_dev::gen_split_kernel<double>, grid, dim3(256), 0, p->stream, spec_out,
+                               (long long)p->ld_out, (long long)p->dist_out, (int)L, tab + p->gtab_2);
+        else
+            hipLaunchKernelGGL(hpxfft_dev::gen_split_kernel<float>, grid, dim3(256), 0, p->stream, spec_out,
+                               (long long)p->ld_out, (long long)p->dist_out, (int)L, tab + p->gtab_2);
+        HIPCHK(hipGetLastError());
+    }
+    return HPXFFT_OK;
+}
+
 // column pass on the spectrum `spec` (in place), or from `spec_in` to
 // `spec_out` for the adjoint; `conj` = inverse transform.
 int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj, int *col_exp,
@@ -504,6 +702,7 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
     a.out_exp = col_exp;
     a.rexp_dist = (long long)p->nx;
     a.cexp_dist = (long long)p->ncols;
+    if (p->g2c.on) return run_columns_2level(p, spec_in, spec_out, conj);
     if (p->gen_col)
     {
         // general length (F64 / F32; in place for the forward transform)
@@ -597,7 +796,11 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     r.in_exp = s16 ? in_exp : nullptr;
     r.out_exp = s16 ? p->mid_exp : nullptr;
     r.exp_dist = (long long)p->nx;
-    if (p->gen_row)
+    if (p->g2r.on)
+    {
+        if (int rc = run_rows_2level(p, d_in, nullptr, nullptr, d_out, false, false)) return rc;
+    }
+    else if (p->gen_row)
     {
         const int ns = p->gen_rows;
         const dim3 grid((unsigned)((p->nx + ns - 1) / ns), (unsigned)p->batch);
@@ -686,39 +889,52 @@ int plan_common(hpxfft_plan_s *p)
     const std::string dims = std::to_string(p->nx) + " x " + std::to_string(p->ny);
     if (p->nx == 0 || p->ny < 2 || p->ny % 2)
         return fail(HPXFFT_UNSUPPORTED_SIZE, "need Nx >= 1 and an even Ny >= 2 (got " + dims + ")");
-    p->gen_row = !is_pow2(p->ny);
-    p->gen_col = !is_pow2(p->nx);
+    // the general kernels take lengths that are not powers of two, and
+    // powers of two beyond the power-of-two kernels (Nx > 2^16, Ny > 2^15)
+    p->gen_row = !is_pow2(p->ny) || p->ny / 2 > (size_t(1) << ROW_MAX_LG);
+    p->gen_col = !is_pow2(p->nx) || p->nx > (size_t(1) << (2 * COL_TILE_MAX_LG));
     if (p->gen_row || p->gen_col)
     {
         // lengths that are not powers of two: general kernels, FP64 / FP32
         if (p->prec != P_F64 && p->prec != P_F32)
-            return fail(HPXFFT_UNSUPPORTED_SIZE, "lengths other than powers of two need an F64 or F32 plan (got " +
-                                                     dims + ")");
+            return fail(HPXFFT_UNSUPPORTED_SIZE, "lengths other than powers of two (or beyond Nx = 65536, Ny = "
+                                                     "32768) need an F64 or F32 plan (got " + dims + ")");
         const size_t cap = p->prec == P_F64 ? hpxfft_dev::gen_max_points<double>()
                                             : hpxfft_dev::gen_max_points<float>();
         auto why = [&](const char *what, size_t n) {
             return std::string(what) + " = " + std::to_string(n) + " needs at most " + std::to_string(cap) +
                    " points with prime factors <= " + std::to_string(hpxfft_dev::GEN_MAX_RADIX) +
                    ", or a Bluestein FFT (the power of two >= 2n-1) of at most " + std::to_string(cap) +
-                   " points, for this precision (got " + dims + ")";
+                   " points, or (up to 2^20 points) a split n = P*Q into two such lengths, for this precision (got " +
+                   dims + ")";
         };
+        const size_t wmax = 128 / (2 * storage_bytes(p->prec));   // one 128-byte segment
         if (p->gen_row && !gen_setup(p->ny / 2, cap, p->grow))
-            return fail(HPXFFT_UNSUPPORTED_SIZE, why("row length Ny/2", p->ny / 2));
+        {
+            if (!gen2_setup(p->ny / 2, cap, p->g2r))
+                return fail(HPXFFT_UNSUPPORTED_SIZE, why("row length Ny/2", p->ny / 2));
+            if (p->nx > 65535 || p->batch > 65535)
+                return fail(HPXFFT_UNSUPPORTED_SIZE, "two-level rows take at most 65535 rows and grids (got " + dims + ")");
+            p->g2r.WA = std::min(gen_width(p->g2r.A, cap, wmax), p->g2r.P);
+            p->g2r.WB = std::min(gen_width(p->g2r.B, cap, wmax), p->g2r.Q);
+        }
         if (p->gen_col && !gen_setup(p->nx, cap, p->gcol))
-            return fail(HPXFFT_UNSUPPORTED_SIZE, why("column length Nx", p->nx));
-        if (p->gen_row)
+        {
+            if (!gen2_setup(p->nx, cap, p->g2c))
+                return fail(HPXFFT_UNSUPPORTED_SIZE, why("column length Nx", p->nx));
+            p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
+            p->g2c.WB = gen_width(p->g2c.B, cap, wmax);
+        }
+        if (p->gen_row && !p->g2r.on)
         {
             // short rows share a workgroup: about 2048 points per workgroup
             const size_t ST = (size_t)p->grow.n;
             p->gen_rows = (int)std::max<size_t>(1, std::min(cap / ST, std::min<size_t>(p->nx, 2048 / ST)));
         }
-        if (p->gen_col)
+        if (p->gen_col && !p->g2c.on)
         {
             // one 128-byte row segment per workgroup where the LDS allows
-            const size_t wmax = 128 / (2 * storage_bytes(p->prec));
-            size_t w = 1;
-            while (w * 2 <= wmax && w * 2 * (size_t)p->gcol.n <= cap) w *= 2;
-            p->gen_w = (int)w;
+            p->gen_w = gen_width(p->gcol, cap, wmax);
         }
     }
     p->lgx = p->gen_col ? 0 : ilog2(p->nx);
@@ -740,26 +956,48 @@ int plan_common(hpxfft_plan_s *p)
     if (p->gen_row || p->gen_col)
     {
         // exact FP64 tables (the long-double evaluation of the power-of-two tables)
-        const size_t L = p->gen_row ? gen_table_size(p->grow) : 0, N2 = p->gen_row ? p->ny : 0;
-        const size_t NX = p->gen_col ? gen_table_size(p->gcol) : 0;
+        const bool row1 = p->gen_row && !p->g2r.on, col1 = p->gen_col && !p->g2c.on;
+        const size_t L = row1 ? gen_table_size(p->grow) : 0, N2 = p->gen_row ? p->ny : 0;
+        const size_t NX = col1 ? gen_table_size(p->gcol) : 0;
         p->gtab_L = 0;
         p->gtab_2 = L;
         p->gtab_N = L + N2;
-        std::vector<double> h(2 * (L + N2 + NX));
-        if (p->gen_row)
-        {
-            gen_table(p->grow, h, p->gtab_L);
-            for (size_t m = 0; m < N2; ++m)   // w_Ny^m for the r2c split
+        size_t total = L + N2 + NX;
+        // two-level dimensions: plans A and B, then w_N^m (m < N) for the phase-A twiddle
+        auto place2 = [&](gen2_dim &d) {
+            if (!d.on) return;
+            d.tabA = total;
+            d.tabB = d.tabA + gen_table_size(d.A);
+            d.tabT = d.tabB + gen_table_size(d.B);
+            total = d.tabT + (size_t)d.P * (size_t)d.Q;
+        };
+        place2(p->g2r);
+        place2(p->g2c);
+        std::vector<double> h(2 * total);
+        auto fill_w = [&](size_t off, size_t n) {   // w_n^m, m < n
+            for (size_t m = 0; m < n; ++m)
             {
                 long double c, sn;
-                twiddle_ld(m, N2, c, sn);
-                h[2 * (p->gtab_2 + m)] = (double)c;
-                h[2 * (p->gtab_2 + m) + 1] = -(double)sn;
+                twiddle_ld(m, n, c, sn);
+                h[2 * (off + m)] = (double)c;
+                h[2 * (off + m) + 1] = -(double)sn;
             }
+        };
+        if (row1) gen_table(p->grow, h, p->gtab_L);
+        if (p->gen_row) fill_w(p->gtab_2, N2);   // w_Ny^m for the r2c split
+        if (col1) gen_table(p->gcol, h, p->gtab_N);
+        for (gen2_dim *d : {&p->g2r, &p->g2c})
+        {
+            if (!d->on) continue;
+            gen_table(d->A, h, d->tabA);
+            gen_table(d->B, h, d->tabB);
+            fill_w(d->tabT, (size_t)d->P * (size_t)d->Q);
         }
-        if (p->gen_col) gen_table(p->gcol, h, p->gtab_N);
         HIPCHK(hipMalloc(&p->gen_tab, sizeof(double) * h.size()));
         HIPCHK(hipMemcpy(p->gen_tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+        const size_t cb = 2 * compute_bytes(p->prec);
+        if (p->g2r.on) HIPCHK(hipMalloc(&p->gtmp, p->batch * p->nx * (p->ny / 2) * cb));
+        if (p->g2c.on) HIPCHK(hipMalloc(&p->gscr, p->batch * p->nx * p->ncols * cb));
     }
     p->W = col_geom(p->prec, p->lgx).nseq;
     const int dmax = col_direct_max_lg(p->prec);
@@ -814,6 +1052,8 @@ void free_plan(hpxfft_plan_s *p)
     (void)hipFree(p->colpart);
     (void)hipFree(p->colmax);
     (void)hipFree(p->gen_tab);
+    (void)hipFree(p->gtmp);
+    (void)hipFree(p->gscr);
     (void)hipFree(p->col_exp_own);
     (void)hipFree(p->adj_work);
     (void)hipFree(p->host_buf);
@@ -994,7 +1234,11 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     r.nrows = (int)plan->nx;
     r.tw = tw_ptr(plan, plan->ny);
     r.tw64 = tw64_ptr(plan, plan->ny);
-    if (plan->gen_row)
+    if (plan->g2r.on)
+    {
+        if (int rc = run_rows_2level(plan, nullptr, d_real, plan->adj_work, nullptr, true, halve)) return rc;
+    }
+    else if (plan->gen_row)
     {
         const int ns = plan->gen_rows;
         const dim3 grid((unsigned)((plan->nx + ns - 1) / ns), (unsigned)plan->batch), block(hpxfft_dev::GEN_NT);

@@ -397,3 +397,140 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_c2r_kernel(RowArgs a, GenPlan 
 }
 
 }   // namespace hpxfft_dev
+
+namespace hpxfft_dev {
+
+// ---------------------------------------------------------------------------
+// Two-level general transforms, for lengths N = P * Q beyond one workgroup:
+// phase A takes the Q-point DFTs over elements m1 + P*e (one sequence per
+// m1), times w_N^{m1 k2}; phase B the P-point DFTs over m1, to element
+// Q*k1 + k2.  gen_strided_kernel runs either phase on W adjacent sequences
+// ("columns") per workgroup, addressed by strides, so the same kernel serves
+// columns of the spectrum and the complex points of a row.
+struct GenStride
+{
+    const void *in;
+    void *out;
+    long long in_seq, in_col, in_sub, in_bat;      // complex-element strides: sequence point, column, blockIdx.y, .z
+    long long out_seq, out_col, out_sub, out_bat;
+    int ncols;      // valid columns
+    int tw_n;       // > 0: output k of column c times w_{tw_n}^{(f k) mod tw_n}, f = c (tw_col) or blockIdx.y
+    int tw_col;
+    int conj_in, conj_out;
+    int pre;        // 1: the input is a half-spectrum row G[0..L] turned into the c2r sequence z (2: interior halved)
+    int prel;       // pre: L
+};
+
+// element (e, w) of a workgroup's W x n block, for i < n W: sequence-fast
+// order when the sequence stride is 1 (contiguous runs), else column-fast
+__device__ __forceinline__ void gen_ew(int i, int n, int W, bool seq_fast, float inv_n, float inv_w, int &e, int &w)
+{
+    if (seq_fast)
+    {
+        w = gdiv(i, n, inv_n);
+        e = i - w * n;
+    }
+    else
+    {
+        e = gdiv(i, W, inv_w);
+        w = i - e * W;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(GEN_NT) gen_strided_kernel(GenStride a, GenPlan g, const cpx<double> *tab,
+                                                            const cpx<double> *twt, const cpx<double> *tab2, int W)
+{
+    cpx<T> *buf = gen_lds<T>();
+    const int t = threadIdx.x, n = g.len, ST = g.n;
+    const int col0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * W;
+    const int nvalid = min(W, a.ncols - col0);
+    const long long sub = blockIdx.y, bat = blockIdx.z;
+    const T *src = reinterpret_cast<const T *>(a.in) + 2 * (bat * a.in_bat + sub * a.in_sub);
+    T *dst = reinterpret_cast<T *>(a.out) + 2 * (bat * a.out_bat + sub * a.out_sub + (long long)col0 * a.out_col);
+    const float inv_n = 1.0f / (float)n, inv_w = 1.0f / (float)W;
+    const bool in_fast = a.in_seq == 1, out_fast = a.out_seq == 1;
+    for (int i = t; i < n * W; i += GEN_NT)
+    {
+        int e, w;
+        gen_ew(i, n, W, in_fast, inv_n, inv_w, e, w);
+        cpx<T> z = {T(0), T(0)};
+        if (w < nvalid)
+        {
+            const long long kk = (long long)e * a.in_seq + (long long)(col0 + w) * a.in_col;
+            if (!a.pre) z = stor<T>::ld(src + 2 * kk);
+            else
+            {
+                // c2r sequence from the half spectrum (as gen_row_c2r_kernel)
+                const int L = a.prel, k = (int)kk;
+                if (k == 0)
+                {
+                    const T h0 = stor<T>::ld(src).x, hl = stor<T>::ld(src + 2 * L).x;
+                    z = cpx<T>{h0 + hl, h0 - hl};
+                }
+                else
+                {
+                    const T hs = a.pre == 2 ? T(0.5) : T(1);
+                    const cpx<T> hk = scal(stor<T>::ld(src + 2 * k), hs), hl = scal(stor<T>::ld(src + 2 * (L - k)), hs);
+                    const cpx<T> wv = tabv<T>(tab2, k);
+                    const cpx<T> wc = {wv.x, -wv.y};
+                    const cpx<T> s0 = hk + conjc(hl), d0 = cmul(hk - conjc(hl), wc);
+                    z = cpx<T>{s0.x - d0.y, s0.y + d0.x};
+                }
+            }
+            if (a.conj_in) z = conjc(z);
+        }
+        buf[w * ST + e] = z;
+    }
+    __syncthreads();
+    const cpx<T> *X = gen_fft<T>(buf, buf + ST * W, W, g, tab);
+    for (int i = t; i < n * W; i += GEN_NT)
+    {
+        int k, w;
+        gen_ew(i, n, W, out_fast, inv_n, inv_w, k, w);
+        if (w >= nvalid) continue;
+        cpx<T> z = X[w * ST + k];
+        if (a.tw_n > 0)
+        {
+            const long long f = a.tw_col ? (long long)(col0 + w) : sub;
+            z = cmul(z, tabv<T>(twt, (int)((f * k) % a.tw_n)));
+        }
+        if (a.conj_out) z = conjc(z);
+        stor<T>::st(dst + 2 * ((long long)k * a.out_seq + (long long)w * a.out_col), z);
+    }
+}
+
+// the r2c split of a row's L-point complex transform Z, in place in the
+// spectrum row (positions 0..L):
+//   X[k] = Fe + w_{Ny}^k Fo, X[L-k] = conj(Fe - w_{Ny}^k Fo), X[0], X[L] from Z[0].
+// blockIdx.y = row, .z = grid; one thread per pair (k, L-k), 2k <= L.
+template <typename T>
+__global__ void __launch_bounds__(256) gen_split_kernel(void *spec, long long row_stride, long long bat_stride, int L,
+                                                         const cpx<double> *tab2)
+{
+    const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (2 * k > L) return;
+    T *o = reinterpret_cast<T *>(spec) + 2 * ((long long)blockIdx.z * bat_stride + (long long)blockIdx.y * row_stride);
+    if (k == 0)
+    {
+        const cpx<T> z0 = stor<T>::ld(o);
+        stor<T>::st(o, cpx<T>{z0.x + z0.y, T(0)});
+        stor<T>::st(o + 2 * L, cpx<T>{z0.x - z0.y, T(0)});
+    }
+    else if (2 * k == L)
+    {
+        stor<T>::st(o + 2 * k, conjc(stor<T>::ld(o + 2 * k)));
+    }
+    else
+    {
+        const cpx<T> A = stor<T>::ld(o + 2 * k), Bc = conjc(stor<T>::ld(o + 2 * (L - k)));
+        const cpx<T> Fe = scal(A + Bc, T(0.5));
+        const cpx<T> D = scal(A - Bc, T(0.5));
+        const cpx<T> Fo = {D.y, -D.x};
+        const cpx<T> tt = cmul(tabv<T>(tab2, k), Fo);
+        stor<T>::st(o + 2 * k, Fe + tt);
+        stor<T>::st(o + 2 * (L - k), conjc(Fe - tt));
+    }
+}
+
+}   // namespace hpxfft_dev

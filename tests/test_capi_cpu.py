@@ -71,13 +71,14 @@ def plan_status(nx, ny, batch=1, ld_in=None, dist_in=None, ld_out=None, dist_out
     return st
 
 
-@pytest.mark.parametrize("nx,ny", [(16, 1), (16, 15), (0, 16), (16, 0), (1 << 17, 4), (4, 1 << 16),
-                                   (16, 2 * 4099), (4099, 16), (16, 2 * 8193), (6000, 8)])
+@pytest.mark.parametrize("nx,ny", [(16, 1), (16, 15), (0, 16), (16, 0), (1 << 21, 4), (4, 1 << 22),
+                                   (16, 2 * 4099), (4099, 16), (16, 2 * 8193), (2 * 4099, 8),
+                                   ((1 << 20) + 2, 4)])
 def test_unsupported_sizes(nx, ny):
-    """Odd Ny, zero sizes, powers of two beyond the kernels' range, and FP64
-    lengths beyond one workgroup's LDS: more than 4096 points, or a prime
-    factor above 61 whose Bluestein FFT (the power of two >= 2n-1) exceeds
-    4096 points (4099 is prime)."""
+    """Odd Ny, zero sizes, and FP64 lengths no plan covers: a prime factor
+    above 61 whose Bluestein FFT (the power of two >= 2n-1) exceeds 4096
+    points and that no split n = P*Q avoids (4099 is prime; 8193 = 3 * 2731),
+    or more than 2^20 points (Nx = 2^21, Ny/2 = 2^21)."""
     assert plan_status(nx, ny) == _lib.UNSUPPORTED_SIZE
     assert hpxfft_amd.last_error()
 
@@ -99,11 +100,13 @@ def test_general_lengths_need_f64_or_f32(prec):
 
 
 @pytest.mark.parametrize("nx,ny", [(8, 14), (3, 16), (1000, 64), (64, 1000), (4095, 2),
-                                   (16, 134), (67, 16), (2039, 2 * 1031)])
+                                   (16, 134), (67, 16), (2039, 2 * 1031),
+                                   (6000, 8), (16, 2 * 6000), (1 << 17, 4), (4, 1 << 17), (3 * 2039, 2)])
 def test_general_lengths_pass_validation(nx, ny):
-    """Lengths that are not powers of two are accepted, prime factors above 61
-    through Bluestein's algorithm (without a device the plan then stops at the
-    device check)."""
+    """Lengths that are not powers of two are accepted: prime factors above 61
+    through Bluestein's algorithm, lengths beyond one workgroup as two levels
+    N = P*Q, as are powers of two beyond the power-of-two kernels (Nx > 2^16,
+    Ny > 2^15) (without a device the plan then stops at the device check)."""
     st = plan_status(nx, ny)
     if st == _lib.OK:
         pytest.skip("a HIP device is present")

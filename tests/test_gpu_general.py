@@ -3,7 +3,8 @@
 The reference accepts any even Ny and any Nx through FFTW; the general
 kernels cover lengths with prime factors <= 61 that fit one workgroup's LDS,
 and other lengths by Bluestein's algorithm (a power-of-two FFT of at least
-2n-1 points in LDS).
+2n-1 points in LDS); longer lengths (and powers of two beyond the
+power-of-two kernels) as two levels N = P*Q of such lengths.
 Checks against the oracle (itself pinned to brute force and numpy for these
 lengths): FP64 <= 1e-12, FP32 <= 1e-5 relative L2, with general rows and/or
 general columns combined with every power-of-two kernel family; batches;
@@ -68,6 +69,85 @@ def test_general_fp32_beyond_fp64_lds(gpu):
     x = oracle.fill(nx, ny, seed=11, dist="normal")
     got = _run(x, nx, ny, "f32")[:, : 2 * (ny // 2 + 1)].copy().view(np.complex128)
     assert oracle.rel_l2(got, oracle.r2c_2d(x, ny)) <= 1e-5
+
+
+# lengths beyond one workgroup: two levels N = P*Q (phase A over m1 + P*e with
+# twiddle w_N^{m1 k2}, phase B over m1), rows then split in place
+SIZES2 = [
+    (6000, 16),       # columns 6000 = 75 * 80
+    (16, 12000),      # rows Ny/2 = 6000
+    (3000, 9000),     # direct columns, two-level rows 4500 = 60 * 75
+    (6117, 6),        # 3 * 2039: a Bluestein phase inside two levels
+    (1 << 17, 4),     # powers of two beyond the power-of-two kernels
+    (4, 1 << 17),
+]
+
+
+@pytest.mark.parametrize("nx,ny", SIZES2)
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5)])
+def test_two_level_parity(gpu, nx, ny, prec, tol):
+    import hpxfft_amd
+    print(f"two-level {prec} {nx}x{ny}", flush=True)
+    x = oracle.fill(nx, ny, seed=nx + 5 * ny, dist="upm1")
+    ref = oracle.r2c_2d(x, ny)
+    got = _run(x, nx, ny, prec)[:, : 2 * (ny // 2 + 1)].copy().view(np.complex128)
+    err = oracle.rel_l2(got, ref)
+    print(f"  rel-L2 = {err:.3e}", flush=True)
+    assert err <= tol, err
+    cap = 4096 if prec == "f64" else 8192     # points of one workgroup's buffer
+
+    def one_level(n):   # direct (prime factors <= 61) or Bluestein within one buffer
+        m, f, big = n, 2, 1
+        while f * f <= m:
+            while m % f == 0:
+                big, m = max(big, f), m // f
+            f += 1
+        big = max(big, m)
+        return (n <= cap and big <= 61) or 1 << (2 * n - 2).bit_length() <= cap
+
+    def two(n, pow2_max):
+        return not (n & (n - 1) == 0 and n <= pow2_max) and not one_level(n)
+
+    p = hpxfft_amd.Plan(nx, ny, precision=prec)
+    assert ("two levels" in p.describe()) == (two(nx, 1 << 16) or two(ny // 2, 1 << 14))
+    p.close()
+
+
+def test_two_level_fp32_bluestein_phase(gpu):
+    """FP32 Bluestein FFTs of up to 8192 points: 8186 = 2 * 4093 (prime) as a
+    2-point phase and a 4093-point Bluestein phase."""
+    nx, ny = 2 * 4093, 8
+    x = oracle.fill(nx, ny, seed=21, dist="normal")
+    got = _run(x, nx, ny, "f32")[:, : 2 * (ny // 2 + 1)].copy().view(np.complex128)
+    assert oracle.rel_l2(got, oracle.r2c_2d(x, ny)) <= 1e-5
+
+
+def test_two_level_batched(gpu):
+    nx, ny, batch = 6000, 2 * 4500, 2
+    ld = 2 * (ny // 2 + 1)
+    xs = [oracle.fill(nx, ny, seed=600 + b, dist="upm1") for b in range(batch)]
+    got = _run(np.concatenate(xs), nx, ny, "f64", batch=batch)
+    for b, x in enumerate(xs):
+        g = got[b * nx:(b + 1) * nx, :ld].copy().view(np.complex128)
+        assert oracle.rel_l2(g, oracle.r2c_2d(x, ny)) <= 1e-12, b
+
+
+@pytest.mark.parametrize("nx,ny", [(6000, 16), (16, 12000), (3000, 9000), (6117, 6)])
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5)])
+def test_two_level_adjoint_and_c2r(gpu, nx, ny, prec, tol):
+    import torch
+    import hpxfft_amd
+    print(f"two-level backward {prec} {nx}x{ny}", flush=True)
+    rng = np.random.default_rng(nx + ny)
+    Y = rng.standard_normal((nx, ny // 2 + 1)) + 1j * rng.standard_normal((nx, ny // 2 + 1))
+    cdt = torch.complex128 if prec == "f64" else torch.complex64
+    Yd = torch.from_numpy(Y).to(device="cuda", dtype=cdt)
+    Yr = Yd.cpu().numpy().astype(np.complex128)
+    adj = hpxfft_amd.rfft2_adjoint(Yd, ny).cpu().numpy().astype(np.float64)
+    assert oracle.rel_l2(adj, oracle.adjoint_r2c_2d(Yr, ny)) <= tol
+    inv = hpxfft_amd.c2r_2d(Yd, ny).cpu().numpy().astype(np.float64)
+    assert oracle.rel_l2(inv, oracle.c2r_2d(Yr, ny)) <= tol
+    hpxfft_amd.clear_plan_cache()
 
 
 def test_bluestein_fp32_beyond_fp64_lds(gpu):
