@@ -12,9 +12,10 @@
 // runtimes/runtimes_hpx_shared_loop.txt (:95-111), same plan file
 // plans/plan_hpx_shared_loop.txt (:114-128).  The n_threads column holds the
 // string "gpu".  Sizes: any Nx and even Ny whose lengths the library
-// supports (powers of two, or lengths within one workgroup's LDS: prime
-// factors <= 61 directly, any other by Bluestein's algorithm); otherwise the
-// class throws std::invalid_argument and the program exits with status 2.
+// supports (powers of two; lengths within one workgroup's LDS, prime factors
+// <= 61 directly and others by Bluestein's algorithm; longer lengths up to
+// 2^20 as a product of two such); otherwise the class throws
+// std::invalid_argument and the program exits with status 2.
 #include "hpxfft/shared/loop.hpp"
 
 #include <cctype>
