@@ -267,7 +267,8 @@ def bench_mixed(n: int = 1 << 15, reps: int = 3) -> dict:
 def bench_general(nx: int = 3000, ny: int = 3000, reps: int = 10) -> dict:
     """Lengths that are not powers of two, FP64, on the general kernels
     (2-pass byte model): by default 3000 x 3000 (radices 4, 2, 3, 5); with
-    prime lengths such as 2039 x 4078 the Bluestein path."""
+    prime lengths such as 2039 x 4078 the Bluestein path; with lengths
+    beyond one workgroup such as 6000 x 12000 the two-level path."""
     import torch
     import hpxfft_amd
 
@@ -411,6 +412,7 @@ def main() -> int:
         result["general_length"] = bench_general()
         progress("general lengths: smooth done")
         result["general_length_bluestein"] = bench_general(2039, 2 * 2039)
+        result["general_length_two_level"] = bench_general(6000, 12000)
         progress("general length done")
         torch.cuda.empty_cache()
     if not args.no_cpu:
