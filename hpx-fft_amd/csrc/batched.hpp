@@ -7,7 +7,7 @@
 
 namespace hpxfft_host {
 
-constexpr int BATCH_NCFG = 3;   // candidate configurations per phase
+constexpr int BATCH_NCFG = 4;   // candidate configurations per phase
 constexpr int BATCH_MIN_SIDE = 64, BATCH_MAX_SIDE = 2048;
 
 // phase 0 = rows, 1 = columns

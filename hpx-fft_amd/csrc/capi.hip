@@ -1556,6 +1556,16 @@ int hpxfft_plan_r2c_2d_batched(hpxfft_plan *plan, size_t count, const size_t *nx
     }
 }
 
+int hpxfft_plan_batched_set_config(hpxfft_plan plan, int row_cfg, int col_cfg)
+{
+    if (!plan || !plan->batched) return fail(HPXFFT_INVALID_ARG, "not a batched plan");
+    if (row_cfg < 0 || row_cfg >= BATCH_NCFG || col_cfg < 0 || col_cfg >= BATCH_NCFG)
+        return fail(HPXFFT_INVALID_ARG, "configuration out of range");
+    plan->bcfg[0] = row_cfg;
+    plan->bcfg[1] = col_cfg;
+    return HPXFFT_OK;
+}
+
 int hpxfft_exec_r2c_2d_batched(hpxfft_plan plan, void *const *d_grids)
 {
     if (!plan || !plan->batched) return fail(HPXFFT_INVALID_ARG, "not a batched plan");

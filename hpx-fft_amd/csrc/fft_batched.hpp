@@ -13,9 +13,12 @@
 //   rows:    CFG 0: PTS 4096, 16 points/thread, 256 threads
 //            CFG 1: PTS 2048,  8 points/thread, 256 threads (radix 8)
 //            CFG 2: PTS 8192, 16 points/thread, 512 threads
+//            CFG 3: PTS 16384, 16 points/thread, 1024 threads
 //   columns: CFG 0: PTS 8192, 16 points/thread, 512 threads
 //            CFG 1: PTS 4096, 16 points/thread, 256 threads
 //            CFG 2: PTS 8192,  8 points/thread, 1024 threads (radix 8)
+//            CFG 3: PTS 16384, 16 points/thread, 1024 threads (a 2048-row
+//                   grid's strip is 8 columns: 64-byte row segments, not 32)
 #pragma once
 
 #include "fft_kernels.hpp"
@@ -41,10 +44,12 @@ template <int CFG> struct brow_cfg;
 template <> struct brow_cfg<0> { static constexpr int PTS = 4096, E = 16, NT = 256; };
 template <> struct brow_cfg<1> { static constexpr int PTS = 2048, E = 8, NT = 256; };
 template <> struct brow_cfg<2> { static constexpr int PTS = 8192, E = 16, NT = 512; };
+template <> struct brow_cfg<3> { static constexpr int PTS = 16384, E = 16, NT = 1024; };
 template <int CFG> struct bcol_cfg;
 template <> struct bcol_cfg<0> { static constexpr int PTS = 8192, E = 16, NT = 512; };
 template <> struct bcol_cfg<1> { static constexpr int PTS = 4096, E = 16, NT = 256; };
 template <> struct bcol_cfg<2> { static constexpr int PTS = 8192, E = 8, NT = 1024; };
+template <> struct bcol_cfg<3> { static constexpr int PTS = 16384, E = 16, NT = 1024; };
 
 // largest LDS footprint over the row lengths of a configuration (complex elements)
 template <int CFG> constexpr int brow_lds()

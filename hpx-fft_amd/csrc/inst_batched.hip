@@ -8,14 +8,16 @@ using namespace hpxfft_dev;
 
 int batched_pts(int phase, int cfg)
 {
-    if (phase == 0) return cfg == 0 ? brow_cfg<0>::PTS : cfg == 1 ? brow_cfg<1>::PTS : brow_cfg<2>::PTS;
-    return cfg == 0 ? bcol_cfg<0>::PTS : cfg == 1 ? bcol_cfg<1>::PTS : bcol_cfg<2>::PTS;
+    if (phase == 0)
+        return cfg == 0 ? brow_cfg<0>::PTS : cfg == 1 ? brow_cfg<1>::PTS : cfg == 2 ? brow_cfg<2>::PTS : brow_cfg<3>::PTS;
+    return cfg == 0 ? bcol_cfg<0>::PTS : cfg == 1 ? bcol_cfg<1>::PTS : cfg == 2 ? bcol_cfg<2>::PTS : bcol_cfg<3>::PTS;
 }
 
 static int nthreads(int phase, int cfg)
 {
-    if (phase == 0) return cfg == 0 ? brow_cfg<0>::NT : cfg == 1 ? brow_cfg<1>::NT : brow_cfg<2>::NT;
-    return cfg == 0 ? bcol_cfg<0>::NT : cfg == 1 ? bcol_cfg<1>::NT : bcol_cfg<2>::NT;
+    if (phase == 0)
+        return cfg == 0 ? brow_cfg<0>::NT : cfg == 1 ? brow_cfg<1>::NT : cfg == 2 ? brow_cfg<2>::NT : brow_cfg<3>::NT;
+    return cfg == 0 ? bcol_cfg<0>::NT : cfg == 1 ? bcol_cfg<1>::NT : cfg == 2 ? bcol_cfg<2>::NT : bcol_cfg<3>::NT;
 }
 
 template <typename K> static int resident(K kern, int nt)
@@ -34,11 +36,13 @@ int batched_resident_blocks(int phase, int cfg)
     {
         if (cfg == 0) return resident(batched_rows_kernel<0>, nt);
         if (cfg == 1) return resident(batched_rows_kernel<1>, nt);
-        return resident(batched_rows_kernel<2>, nt);
+        if (cfg == 2) return resident(batched_rows_kernel<2>, nt);
+        return resident(batched_rows_kernel<3>, nt);
     }
     if (cfg == 0) return resident(batched_cols_kernel<0>, nt);
     if (cfg == 1) return resident(batched_cols_kernel<1>, nt);
-    return resident(batched_cols_kernel<2>, nt);
+    if (cfg == 2) return resident(batched_cols_kernel<2>, nt);
+    return resident(batched_cols_kernel<3>, nt);
 }
 
 hipError_t batched_launch(int phase, int cfg, const void *grids, const void *tasks, int ntasks, unsigned *ticket,
@@ -51,13 +55,15 @@ hipError_t batched_launch(int phase, int cfg, const void *grids, const void *tas
     {
         if (cfg == 0) hipLaunchKernelGGL(batched_rows_kernel<0>, grid, block, 0, s, g, t, ntasks, ticket, tw32);
         else if (cfg == 1) hipLaunchKernelGGL(batched_rows_kernel<1>, grid, block, 0, s, g, t, ntasks, ticket, tw32);
-        else hipLaunchKernelGGL(batched_rows_kernel<2>, grid, block, 0, s, g, t, ntasks, ticket, tw32);
+        else if (cfg == 2) hipLaunchKernelGGL(batched_rows_kernel<2>, grid, block, 0, s, g, t, ntasks, ticket, tw32);
+        else hipLaunchKernelGGL(batched_rows_kernel<3>, grid, block, 0, s, g, t, ntasks, ticket, tw32);
     }
     else
     {
         if (cfg == 0) hipLaunchKernelGGL(batched_cols_kernel<0>, grid, block, 0, s, g, t, ntasks, ticket, tw32);
         else if (cfg == 1) hipLaunchKernelGGL(batched_cols_kernel<1>, grid, block, 0, s, g, t, ntasks, ticket, tw32);
-        else hipLaunchKernelGGL(batched_cols_kernel<2>, grid, block, 0, s, g, t, ntasks, ticket, tw32);
+        else if (cfg == 2) hipLaunchKernelGGL(batched_cols_kernel<2>, grid, block, 0, s, g, t, ntasks, ticket, tw32);
+        else hipLaunchKernelGGL(batched_cols_kernel<3>, grid, block, 0, s, g, t, ntasks, ticket, tw32);
     }
     return hipGetLastError();
 }

@@ -133,6 +133,10 @@ class BatchedPlan:
             self._ptrs[i] = p
         check(lib().hpxfft_exec_r2c_2d_batched(self._h, self._ptrs))
 
+    def set_config(self, row_cfg: int, col_cfg: int) -> None:
+        """Run the given task shapes (0..3) instead of the chosen ones."""
+        check(lib().hpxfft_plan_batched_set_config(self._h, int(row_cfg), int(col_cfg)))
+
     measurement = Plan.measurement
     describe = Plan.describe
     close = Plan.close

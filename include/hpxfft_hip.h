@@ -158,6 +158,12 @@ int hpxfft_plan_r2c_2d_batched(hpxfft_plan *plan, size_t count, const size_t *nx
 /* d_grids: host array of `count` device pointers, one per grid (in place). */
 int hpxfft_exec_r2c_2d_batched(hpxfft_plan plan, void *const *d_grids);
 
+/* No reference counterpart: overrides the task shape a batched plan runs in
+ * each phase (row_cfg, col_cfg in 0..3: the candidates its autotuner times;
+ * the plan text lists them).  HPXFFT_INVALID_ARG for another value or a plan
+ * that is not batched. */
+int hpxfft_plan_batched_set_config(hpxfft_plan plan, int row_cfg, int col_cfg);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *hpxfft_last_error(void);
 

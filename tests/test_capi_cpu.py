@@ -130,6 +130,7 @@ def test_bad_arguments():
     v = ctypes.c_double()
     assert L.hpxfft_get_measurement(None, b"total", ctypes.byref(v)) == _lib.INVALID_ARG
     assert L.hpxfft_write_plan(None, b"/tmp/x") == _lib.INVALID_ARG
+    assert L.hpxfft_plan_batched_set_config(None, 0, 0) == _lib.INVALID_ARG
     assert L.hpxfft_plan_destroy(None) == _lib.OK
     eff = ctypes.c_int()
     assert L.hpxfft_plan_flag_from_string(None, ctypes.byref(eff)) == _lib.INVALID_ARG

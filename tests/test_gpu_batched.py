@@ -61,3 +61,38 @@ def test_batched_matches_single_grid_plans(gpu):
     torch.cuda.synchronize()
     for x, y in zip(xs, ys):
         assert (torch.linalg.vector_norm(x - y) / torch.linalg.vector_norm(y)).item() <= 1e-5
+
+
+@pytest.mark.parametrize("cfg", [0, 1, 2, 3])
+def test_batched_every_configuration(gpu, cfg):
+    """Each candidate task shape (row and column phase), forced, against the
+    oracle on grids that include both extremes of each side."""
+    import torch
+    import hpxfft_amd
+    nx, ny = make_sizes(10, 11)
+    nx += [64, 2048, 2048, 64]
+    ny += [64, 2048, 64, 2048]
+    xs = [oracle.fill(a, b, seed=3000 + i, dist="normal") for i, (a, b) in enumerate(zip(nx, ny))]
+    ds = [torch.from_numpy(x).to(device=gpu, dtype=torch.float32) for x in xs]
+    plan = hpxfft_amd.BatchedPlan(nx, ny)
+    plan.set_config(cfg, cfg)
+    assert f"configuration {cfg} " in plan.describe()
+    plan.exec([d.data_ptr() for d in ds])
+    torch.cuda.synchronize()
+    plan.close()
+    for i, (a, b) in enumerate(zip(nx, ny)):
+        got = ds[i].cpu().numpy().astype(np.float64).view(np.complex128)
+        assert oracle.rel_l2(got, oracle.r2c_2d(xs[i], b)) <= 1e-5, (cfg, a, b)
+
+
+def test_batched_set_config_rejections(gpu):
+    import hpxfft_amd
+    plan = hpxfft_amd.BatchedPlan([64], [64])
+    for bad in [(-1, 0), (0, 4), (4, 4)]:
+        with pytest.raises(RuntimeError, match="out of range"):
+            plan.set_config(*bad)
+    plan.close()
+    p = hpxfft_amd.Plan(64, 64)
+    h = p.handle
+    assert hpxfft_amd.lib().hpxfft_plan_batched_set_config(h, 0, 0) != 0
+    p.close()
