@@ -278,7 +278,7 @@ def bench_general(nx: int = 3000, ny: int = 3000, reps: int = 10) -> dict:
     x[:, :ny].uniform_(-1.0, 1.0, generator=g)
     x[:, ny:].zero_()
     out = torch.empty((nx, ny // 2 + 1), dtype=torch.complex128, device="cuda")
-    p = hpxfft_amd.Plan(nx, ny, ld_in=ld, precision="f64")
+    p = hpxfft_amd.Plan(nx, ny, ld_in=ld, precision="f64", effort="measure")
     ms = _events_ms(lambda: p.exec(x.data_ptr(), out.data_ptr()), reps)
     desc = [ln.strip() for ln in p.describe().splitlines() if "general" in ln]
     p.close()

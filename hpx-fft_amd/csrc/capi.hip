@@ -177,7 +177,8 @@ const char *prec_name(int prec)
 // Q-point plan A over elements m1 + P*e, phase B the P-point plan B
 struct gen2_dim
 {
-    bool on = false;
+    bool on = false;       // the dimension runs as two levels
+    bool avail = false;    // tables and scratch in place (on, or a candidate the measure effort times)
     int P = 0, Q = 0;
     hpxfft_dev::GenPlan A{}, B{};
     int WA = 1, WB = 1;                    // sequences per workgroup in each phase
@@ -206,6 +207,7 @@ struct hpxfft_plan_s
     gen2_dim g2r, g2c;             // two-level general rows (Ny/2 = P*Q) / columns (Nx = P*Q)
     void *gtmp = nullptr;          // two-level rows: phase A output, Nx x Ny/2 per grid (compute type)
     void *gscr = nullptr;          // two-level columns: phase A output, Nx x (Ny/2+1) per grid
+    double gcol_ms[2] = {};        // measure effort: general column pass ms, one level / two levels
     double *gen_tab = nullptr;     // exact FP64 tables w_n^m: rows L = Ny/2, split Ny, columns Nx
     size_t gtab_L = 0, gtab_2 = 0, gtab_N = 0;   // their offsets (complex entries)
     // device memory
@@ -349,7 +351,7 @@ bool gen2_setup(size_t N, size_t cap, gen2_dim &d)
         }
     }
     if (best < 0) return false;
-    d.on = true;
+    d.on = d.avail = true;
     d.P = best;
     d.Q = (int)(N / (size_t)best);
     return true;
@@ -502,6 +504,9 @@ std::string describe(const hpxfft_plan_s *p)
            << rg.nseq << " row(s)/workgroup, " << rg.nt << " threads, " << rg.E << " elements/thread, radix "
            << (rg.E < 16 ? rg.E : 16) << (rg.split ? ", LDS exchange in re/im halves" : "") << "\n";
     }
+    if (p->gcol_ms[0] > 0.0)
+        os << "  general column pass autotuned (ms): one level " << p->gcol_ms[0] << ", two levels " << p->gcol_ms[1]
+           << "\n";
     if (p->g2c.on) two_level("column c2c", p->g2c);
     else if (p->gen_col)
         os << "  column c2c kernel (general length): " << p->nx << "-point " << gen_method(p->gcol) << ", "
@@ -884,6 +889,46 @@ int tune_split(hpxfft_plan_s *p)
     return HPXFFT_OK;
 }
 
+// measure effort, general columns with narrow strips: time the one-level
+// and the two-level column pass on a zeroed spectrum of the plan's shape and
+// keep the faster (the scratch of the two-level form is freed if unused)
+int tune_gen_columns(hpxfft_plan_s *p)
+{
+    const size_t bytes = p->batch * p->dist_out * 2 * storage_bytes(p->prec);
+    void *tmp = nullptr;
+    if (hipMalloc(&tmp, bytes) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        p->g2c.on = false;
+        return HPXFFT_OK;
+    }
+    int rc = HPXFFT_OK;
+    hipError_t e = hipMemsetAsync(tmp, 0, bytes, p->stream);
+    for (int two = 0; two < 2 && e == hipSuccess && rc == HPXFFT_OK; ++two)
+    {
+        p->g2c.on = two == 1;
+        constexpr int REPS = 3;
+        rc = run_columns(p, tmp, tmp, false, nullptr);   // warm-up
+        if (rc == HPXFFT_OK) e = hipEventRecord(p->ev[0], p->stream);
+        for (int r = 0; r < REPS && rc == HPXFFT_OK && e == hipSuccess; ++r) rc = run_columns(p, tmp, tmp, false, nullptr);
+        if (rc == HPXFFT_OK && e == hipSuccess) e = hipEventRecord(p->ev[2], p->stream);
+        if (rc == HPXFFT_OK && e == hipSuccess) e = hipEventSynchronize(p->ev[2]);
+        if (rc == HPXFFT_OK && e == hipSuccess) p->gcol_ms[two] = elapsed(p->ev[0], p->ev[2]) * 1e3 / REPS;
+    }
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipFree(tmp);
+    p->g2c.on = p->gcol_ms[1] > 0.0 && p->gcol_ms[1] < p->gcol_ms[0];
+    if (!p->g2c.on)
+    {
+        (void)hipFree(p->gscr);
+        p->gscr = nullptr;
+        p->g2c.avail = false;
+    }
+    if (rc != HPXFFT_OK) return rc;
+    HIPCHK(e);
+    return HPXFFT_OK;
+}
+
 int plan_common(hpxfft_plan_s *p)
 {
     const std::string dims = std::to_string(p->nx) + " x " + std::to_string(p->ny);
@@ -935,6 +980,14 @@ int plan_common(hpxfft_plan_s *p)
         {
             // one 128-byte row segment per workgroup where the LDS allows
             p->gen_w = gen_width(p->gcol, cap, wmax);
+            // narrow strips (16- or 32-byte row segments): at measure effort the
+            // two-level form, with whole segments, is timed against it
+            if (p->effort >= HPXFFT_MEASURE && p->gen_w <= 2 && gen2_setup(p->nx, cap, p->g2c))
+            {
+                p->g2c.on = false;
+                p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
+                p->g2c.WB = gen_width(p->g2c.B, cap, wmax);
+            }
         }
     }
     p->lgx = p->gen_col ? 0 : ilog2(p->nx);
@@ -965,7 +1018,7 @@ int plan_common(hpxfft_plan_s *p)
         size_t total = L + N2 + NX;
         // two-level dimensions: plans A and B, then w_N^m (m < N) for the phase-A twiddle
         auto place2 = [&](gen2_dim &d) {
-            if (!d.on) return;
+            if (!d.avail) return;
             d.tabA = total;
             d.tabB = d.tabA + gen_table_size(d.A);
             d.tabT = d.tabB + gen_table_size(d.B);
@@ -988,7 +1041,7 @@ int plan_common(hpxfft_plan_s *p)
         if (col1) gen_table(p->gcol, h, p->gtab_N);
         for (gen2_dim *d : {&p->g2r, &p->g2c})
         {
-            if (!d->on) continue;
+            if (!d->avail) continue;
             gen_table(d->A, h, d->tabA);
             gen_table(d->B, h, d->tabB);
             fill_w(d->tabT, (size_t)d->P * (size_t)d->Q);
@@ -997,7 +1050,7 @@ int plan_common(hpxfft_plan_s *p)
         HIPCHK(hipMemcpy(p->gen_tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
         const size_t cb = 2 * compute_bytes(p->prec);
         if (p->g2r.on) HIPCHK(hipMalloc(&p->gtmp, p->batch * p->nx * (p->ny / 2) * cb));
-        if (p->g2c.on) HIPCHK(hipMalloc(&p->gscr, p->batch * p->nx * p->ncols * cb));
+        if (p->g2c.avail) HIPCHK(hipMalloc(&p->gscr, p->batch * p->nx * p->ncols * cb));
     }
     p->W = col_geom(p->prec, p->lgx).nseq;
     const int dmax = col_direct_max_lg(p->prec);
@@ -1032,6 +1085,8 @@ int plan_common(hpxfft_plan_s *p)
     if (int rc = alloc_events(p)) return rc;
     if (p->four_step && p->effort >= HPXFFT_MEASURE)
         if (int rc = tune_split(p)) return rc;
+    if (p->g2c.avail && !p->g2c.on)
+        if (int rc = tune_gen_columns(p)) return rc;
     const double lx = std::log2((double)p->nx), ly = std::log2((double)p->ny);
     p->plan_flops = (double)p->batch * ((double)p->nx * 2.5 * (double)p->ny * ly +
                                         (double)p->ncols * 5.0 * (double)p->nx * lx);

@@ -113,6 +113,27 @@ def test_two_level_parity(gpu, nx, ny, prec, tol):
     p.close()
 
 
+@pytest.mark.parametrize("nx,ny,prec,tol", [(6144, 64, "f32", 1e-5), (3000, 32, "f64", 1e-12),
+                                             (2500, 20, "f64", 1e-12)])
+def test_measure_times_one_and_two_level_columns(gpu, nx, ny, prec, tol):
+    """Narrow general column strips (one 16- or 32-byte segment per row): the
+    measure effort times the one-level pass against the two-level form and
+    keeps the faster; either way the result is the oracle's."""
+    import torch
+    import hpxfft_amd
+    x = oracle.fill(nx, ny, seed=nx + ny, dist="upm1")
+    dt = torch.float64 if prec == "f64" else torch.float32
+    d = torch.from_numpy(x).to(device="cuda", dtype=dt)
+    p = hpxfft_amd.Plan(nx, ny, precision=prec, effort="measure")
+    desc = p.describe()
+    assert "general column pass autotuned" in desc, desc
+    p.exec(d.data_ptr(), d.data_ptr())
+    torch.cuda.synchronize()
+    p.close()
+    got = d.cpu().numpy().astype(np.float64)[:, : 2 * (ny // 2 + 1)].copy().view(np.complex128)
+    assert oracle.rel_l2(got, oracle.r2c_2d(x, ny)) <= tol
+
+
 def test_two_level_fp32_bluestein_phase(gpu):
     """FP32 Bluestein FFTs of up to 8192 points: 8186 = 2 * 4093 (prime) as a
     2-point phase and a 4093-point Bluestein phase."""
