@@ -230,38 +230,45 @@ def bench_mixed(n: int = 1 << 15, reps: int = 3) -> dict:
     x64[:, n:].zero_()
     out = {}
     ref = torch.empty((n, n // 2 + 1), dtype=torch.complex128, device="cuda")
-    p64 = hpxfft_amd.Plan(n, n, ld_in=ld, precision="f64")
+    p64 = hpxfft_amd.Plan(n, n, ld_in=ld, precision="f64", effort="measure")
     p64.exec(x64.data_ptr(), ref.data_ptr())
     out["f64_ms"] = round(_events_ms(lambda: p64.exec(x64.data_ptr(), ref.data_ptr()), reps), 3)
     p64.close()
     nref = torch.linalg.vector_norm(ref)
     x32 = x64.float()
     y32 = torch.empty((n, n // 2 + 1), dtype=torch.complex64, device="cuda")
-    p32 = hpxfft_amd.Plan(n, n, ld_in=ld, precision="f32")
+    p32 = hpxfft_amd.Plan(n, n, ld_in=ld, precision="f32", effort="measure")
     t32 = _events_ms(lambda: p32.exec(x32.data_ptr(), y32.data_ptr()), reps)
+    split32 = _split_line(p32)
     p32.close()
     err = (torch.linalg.vector_norm(y32.to(torch.complex128) - ref) / nref).item()
     out["f32"] = {"ms": round(t32, 3), "rel_l2_vs_f64": err,
-                  "GBps": round(alg_bytes(n, n, 4)["total"] / (t32 * 1e-3) / 1e9, 1)}
+                  "GBps": round(alg_bytes(n, n, 4)["total"] / (t32 * 1e-3) / 1e9, 1), "column_pass": split32}
     del x32, y32
     for name, dt, prec in (("f16", torch.float16, "f16"), ("bf16", torch.bfloat16, "bf16")):
         x16 = x64.to(dt)
         buf = torch.empty_like(x16)
         col_exp = torch.empty(n // 2 + 1, dtype=torch.int32, device="cuda")
-        p = hpxfft_amd.Plan(n, n, ld_in=ld, ld_out=ld // 2, precision=prec)
+        p = hpxfft_amd.Plan(n, n, ld_in=ld, ld_out=ld // 2, precision=prec, effort="measure")
         # the copy alone first: the timed loop must be the last writer of buf
         tc = _events_ms(lambda: buf.copy_(x16), reps)
         t = _events_ms(lambda: (buf.copy_(x16), p.exec_scaled(buf.data_ptr(), buf.data_ptr(), None,
                                                               col_exp.data_ptr())), reps)
+        split = _split_line(p)
         p.close()
         Xs = buf.view(n, ld // 2, 2).float()
         X = torch.complex(Xs[..., 0], Xs[..., 1]).to(torch.complex128) * torch.exp2(-col_exp.double())[None, :]
         e = (torch.linalg.vector_norm(X - ref) / nref).item()
         out[name] = {"ms": round(t - tc, 3), "rel_l2_vs_f64": e,
                      "GBps": round(alg_bytes(n, n, 2)["total"] / ((t - tc) * 1e-3) / 1e9, 1),
-                     "scales": "per input row after the row pass, per output column after the column pass"}
+                     "scales": "per input row after the row pass, per output column after the column pass",
+                     "column_pass": split}
         del x16, buf, Xs, X
     return out
+
+
+def _split_line(p) -> list:
+    return [ln.strip() for ln in p.describe().splitlines() if "four-step" in ln or "autotuned" in ln]
 
 
 def bench_general(nx: int = 3000, ny: int = 3000, reps: int = 10) -> dict:
