@@ -308,13 +308,20 @@ tile_kernel(ColArgs a)
             cpx<T> z = v[i * R1 + q];
             if constexpr (MODE == 1)
             {
-                stor<SOUT>::st(gout + 2 * (sbase + ((long long)e * a.P + sub) * W + w), z);
+                // FP64: streaming stores of the scratch and (below) of the
+                // spectrum measured 1.4% faster over the three launches
+                if constexpr (sizeof(SS) == 8) st_stream<SOUT>(gout + 2 * (sbase + ((long long)e * a.P + sub) * W + w), z);
+                else stor<SOUT>::st(gout + 2 * (sbase + ((long long)e * a.P + sub) * W + w), z);
             }
             else
             {
                 if constexpr (CONJ) z = conjc(z);
                 if constexpr (S16) z = scal(z, osc);
-                if (valid) stor<SOUT>::st(gout + 2 * ((long long)(a.Q * e + sub) * a.pitch + c), z);
+                if (valid)
+                {
+                    if constexpr (sizeof(SS) == 8) st_stream<SOUT>(gout + 2 * ((long long)(a.Q * e + sub) * a.pitch + c), z);
+                    else stor<SOUT>::st(gout + 2 * ((long long)(a.Q * e + sub) * a.pitch + c), z);
+                }
             }
         }
     }
