@@ -106,13 +106,21 @@ int hpxfft_exec_r2c_2d_scaled(hpxfft_plan plan, const void *d_in, void *d_out, c
 /* Drop-in host path for fft_2d_r2c_par (loop.cpp:56-113) on the reference's
  * in-place padded buffer: copies `inout` (Nx x ld_in doubles) to the device,
  * transforms, copies back and synchronises.  F64 plans only.  Adds the
- * measurement keys "h2d" and "d2h". */
+ * measurement keys "h2d" and "d2h".  The first call allocates the plan's
+ * device copy of the buffer. */
 int hpxfft_exec_r2c_2d_host(hpxfft_plan plan, double *inout);
 
 /* Adjoint of the r2c map under Re<A,B> on the half spectrum (no reference
  * counterpart):  (R* Y)[m][n] = Re sum_kx sum_{ky<=Ny/2} Y[kx][ky]
  * exp(+2 pi i (kx m/Nx + ky n/Ny)).  d_spec uses the plan's output layout
- * and is not modified; d_real uses the plan's input layout.  F64/F32 plans. */
+ * and is not modified; d_real uses the plan's input layout.  F64/F32 plans.
+ * The first backward call on a plan (this or hpxfft_exec_c2r_2d) allocates
+ * its work buffer, one spectrum; later calls allocate nothing.
+ *
+ * Every exec function enqueues only kernels and event records on the plan's
+ * stream, so a caller may capture it into a HIP graph once that first
+ * allocation is done (make one backward call before capturing a backward
+ * one). */
 int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_real);
 
 /* Inverse 2-D c2r, unnormalised, in FFTW's FFTW_BACKWARD convention: the
