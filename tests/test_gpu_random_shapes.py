@@ -1,13 +1,14 @@
 """Seeded random grid shapes on the GPU through the C ABI, against the oracle.
 
 Forty shapes (Nx * Ny <= 1.5e6) mixing powers of two with arbitrary lengths
-up to 3000 (prime lengths among them), so the power-of-two kernels, the
-general one-level kernels, Bluestein's algorithm and the two-level path all
-meet.  For each shape the plan must either be created, and then match the
-oracle (FP64 <= 1e-12, FP32 <= 1e-5 relative L2, forward; the adjoint on
-every fourth shape), or fail with UNSUPPORTED_SIZE exactly when no plan
-covers a length: a prime factor above 61 whose Bluestein FFT exceeds one
-workgroup's buffer and that no split N = P*Q avoids.
+up to 3000 (prime lengths among them), and ten with one side of 4097 to
+20000 points, so the power-of-two kernels, the general one-level kernels,
+Bluestein's algorithm and the two-level path all meet.  For each shape the
+plan must either be created, and then match the oracle (FP64 <= 1e-12,
+FP32 <= 1e-5 relative L2, forward; the adjoint on every fourth shape), or
+fail with UNSUPPORTED_SIZE exactly when no plan covers a length: a prime
+factor above 61 whose Bluestein FFT exceeds one workgroup's buffer and that
+no split N = P*Q avoids.
 """
 import numpy as np
 import pytest
@@ -29,7 +30,18 @@ def _shapes():
     return out
 
 
-SHAPES = _shapes()
+def _long_shapes():
+    """Ten shapes with one side beyond one workgroup (4097..20000)."""
+    rng = np.random.default_rng(77)
+    out = []
+    for i in range(10):
+        long_side = int(rng.integers(4097, 20001))
+        short = int(rng.integers(1, 64))
+        out.append((long_side, 2 * short) if i % 2 == 0 else (short, 2 * long_side))
+    return out
+
+
+SHAPES = _shapes() + _long_shapes()
 
 
 def _largest_prime(n):
