@@ -171,14 +171,18 @@ def test_adjoint_torch_fixtures_and_brute(nx, ny):
     assert np.max(np.abs(oracle.adjoint_brute(Y, ny) - ref)) < 1e-12 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("n", [3, 5, 6, 7, 9, 12, 14, 15, 30, 49, 97, 210, 1155])
+@pytest.mark.parametrize("n", [3, 5, 6, 7, 9, 12, 14, 15, 30, 49, 97, 210, 1155, 1031, 2039, 4099, 6000, 8186,
+                               131072])
 def test_general_length_c2c_matches_numpy(n):
-    """Mixed-radix lengths (any n): forward and backward against numpy.fft."""
+    """Mixed-radix lengths (any n, primes up to 4099 among them, the lengths
+    the GPU's Bluestein and two-level paths take): forward and backward
+    against numpy.fft."""
     rng = np.random.default_rng(n)
     x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
     ref = np.fft.fft(x)
-    assert np.max(np.abs(oracle.fft_c2c_1d(x) - ref)) <= 1e-14 * np.max(np.abs(ref))
-    assert np.max(np.abs(oracle.fft_c2c_1d(x, +1) - n * np.fft.ifft(x))) <= 1e-14 * np.max(np.abs(ref))
+    tol = 1e-14 if n < 1024 else 1e-13
+    assert np.max(np.abs(oracle.fft_c2c_1d(x) - ref)) <= tol * np.max(np.abs(ref))
+    assert np.max(np.abs(oracle.fft_c2c_1d(x, +1) - n * np.fft.ifft(x))) <= tol * np.max(np.abs(ref))
 
 
 @pytest.mark.parametrize("nx,ny", [(8, 14), (3, 10), (5, 6), (7, 30), (6, 4), (1, 2), (9, 18)])
