@@ -253,8 +253,12 @@ const void *tw_ptr(const hpxfft_plan_s *p, size_t N)
 }
 const void *tw64_ptr(const hpxfft_plan_s *p, size_t N) { return p->tabs.tw64 + 2 * N; }
 
-// dynamic LDS of a general kernel: two buffers of `points` complex values in the compute type
-size_t gen_lds_bytes(const hpxfft_plan_s *p, size_t points) { return 2 * points * 2 * compute_bytes(p->prec); }
+// dynamic LDS of a general kernel: two buffers of `points` complex values in
+// the compute type, then `words` 32-bit words (16-bit storage: row / column maxima)
+size_t gen_lds_bytes(const hpxfft_plan_s *p, size_t points, size_t words = 0)
+{
+    return 2 * points * 2 * compute_bytes(p->prec) + 4 * words;
+}
 
 // let every general kernel take up to GEN_LDS_BYTES of dynamic LDS (above the
 // 64 KiB default); once per device
@@ -267,12 +271,20 @@ int gen_allow_lds(int dev)
     using namespace hpxfft_dev;
     const void *fns[] = {
         reinterpret_cast<const void *>(&gen_row_r2c_kernel<double>), reinterpret_cast<const void *>(&gen_row_r2c_kernel<float>),
-        reinterpret_cast<const void *>(&gen_col_kernel<double, false>), reinterpret_cast<const void *>(&gen_col_kernel<double, true>),
-        reinterpret_cast<const void *>(&gen_col_kernel<float, false>), reinterpret_cast<const void *>(&gen_col_kernel<float, true>),
+        reinterpret_cast<const void *>(&gen_row_r2c_kernel<float, __half>),
+        reinterpret_cast<const void *>(&gen_row_r2c_kernel<float, bf16_t>),
+        reinterpret_cast<const void *>(&gen_col_kernel<double, double, false>),
+        reinterpret_cast<const void *>(&gen_col_kernel<double, double, true>),
+        reinterpret_cast<const void *>(&gen_col_kernel<float, float, false>),
+        reinterpret_cast<const void *>(&gen_col_kernel<float, float, true>),
+        reinterpret_cast<const void *>(&gen_col_kernel<float, __half, false>),
+        reinterpret_cast<const void *>(&gen_col_kernel<float, bf16_t, false>),
         reinterpret_cast<const void *>(&gen_row_c2r_kernel<double, false>), reinterpret_cast<const void *>(&gen_row_c2r_kernel<double, true>),
         reinterpret_cast<const void *>(&gen_row_c2r_kernel<float, false>), reinterpret_cast<const void *>(&gen_row_c2r_kernel<float, true>),
         reinterpret_cast<const void *>(&gen_strided_kernel<double>), reinterpret_cast<const void *>(&gen_strided_kernel<float>)};
-    for (const void *f : fns) HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, GEN_LDS_BYTES));
+    // the buffers plus up to 1024 words of per-row / per-column maxima
+    for (const void *f : fns)
+        HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, GEN_LDS_BYTES + 4096));
     done[dev] = true;
     return HPXFFT_OK;
 }
@@ -715,17 +727,29 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         a.out = spec_out;
         const dim3 grid((unsigned)((p->ncols + p->gen_w - 1) / p->gen_w), (unsigned)p->batch);
         const dim3 block(hpxfft_dev::GEN_NT);
-        const size_t lds = gen_lds_bytes(p, (size_t)p->gen_w * p->gcol.n);
+        const size_t lds = gen_lds_bytes(p, (size_t)p->gen_w * p->gcol.n, (size_t)p->gen_w);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab) + p->gtab_N;
-        if (p->prec == P_F64)
+        using namespace hpxfft_dev;
+        const GenPlan &g = p->gcol;
+        const int W = p->gen_w;
+        switch (p->prec)
         {
-            if (conj) hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<double, true>), grid, block, lds, p->stream, a, p->gcol, tab, p->gen_w);
-            else hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<double, false>), grid, block, lds, p->stream, a, p->gcol, tab, p->gen_w);
-        }
-        else
-        {
-            if (conj) hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<float, true>), grid, block, lds, p->stream, a, p->gcol, tab, p->gen_w);
-            else hipLaunchKernelGGL((hpxfft_dev::gen_col_kernel<float, false>), grid, block, lds, p->stream, a, p->gcol, tab, p->gen_w);
+        case P_F64:
+            if (conj) hipLaunchKernelGGL((gen_col_kernel<double, double, true>), grid, block, lds, p->stream, a, g, tab, W);
+            else hipLaunchKernelGGL((gen_col_kernel<double, double, false>), grid, block, lds, p->stream, a, g, tab, W);
+            break;
+        case P_F32:
+            if (conj) hipLaunchKernelGGL((gen_col_kernel<float, float, true>), grid, block, lds, p->stream, a, g, tab, W);
+            else hipLaunchKernelGGL((gen_col_kernel<float, float, false>), grid, block, lds, p->stream, a, g, tab, W);
+            break;
+        case P_F16:
+            if (conj) return fail(HPXFFT_INVALID_ARG, "16-bit storage plans are forward only");
+            hipLaunchKernelGGL((gen_col_kernel<float, __half, false>), grid, block, lds, p->stream, a, g, tab, W);
+            break;
+        default:
+            if (conj) return fail(HPXFFT_INVALID_ARG, "16-bit storage plans are forward only");
+            hipLaunchKernelGGL((gen_col_kernel<float, bf16_t, false>), grid, block, lds, p->stream, a, g, tab, W);
+            break;
         }
         HIPCHK(hipGetLastError());
         return HPXFFT_OK;
@@ -808,15 +832,26 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     else if (p->gen_row)
     {
         const int ns = p->gen_rows;
-        const dim3 grid((unsigned)((p->nx + ns - 1) / ns), (unsigned)p->batch);
-        const size_t lds = gen_lds_bytes(p, (size_t)ns * p->grow.n);
+        const dim3 grid((unsigned)((p->nx + ns - 1) / ns), (unsigned)p->batch), block(hpxfft_dev::GEN_NT);
+        const size_t lds = gen_lds_bytes(p, (size_t)ns * p->grow.n, (size_t)ns);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
-        if (p->prec == P_F64)
-            hipLaunchKernelGGL(hpxfft_dev::gen_row_r2c_kernel<double>, grid, dim3(hpxfft_dev::GEN_NT), lds, p->stream,
-                               r, p->grow, tab + p->gtab_L, tab + p->gtab_2, ns);
-        else
-            hipLaunchKernelGGL(hpxfft_dev::gen_row_r2c_kernel<float>, grid, dim3(hpxfft_dev::GEN_NT), lds, p->stream,
-                               r, p->grow, tab + p->gtab_L, tab + p->gtab_2, ns);
+        const auto *tL = tab + p->gtab_L, *t2 = tab + p->gtab_2;
+        using namespace hpxfft_dev;
+        switch (p->prec)
+        {
+        case P_F64:
+            hipLaunchKernelGGL((gen_row_r2c_kernel<double, double>), grid, block, lds, p->stream, r, p->grow, tL, t2, ns);
+            break;
+        case P_F32:
+            hipLaunchKernelGGL((gen_row_r2c_kernel<float, float>), grid, block, lds, p->stream, r, p->grow, tL, t2, ns);
+            break;
+        case P_F16:
+            hipLaunchKernelGGL((gen_row_r2c_kernel<float, __half>), grid, block, lds, p->stream, r, p->grow, tL, t2, ns);
+            break;
+        default:
+            hipLaunchKernelGGL((gen_row_r2c_kernel<float, bf16_t>), grid, block, lds, p->stream, r, p->grow, tL, t2, ns);
+            break;
+        }
         HIPCHK(hipGetLastError());
     }
     else
@@ -940,10 +975,9 @@ int plan_common(hpxfft_plan_s *p)
     p->gen_col = !is_pow2(p->nx) || p->nx > (size_t(1) << (2 * COL_TILE_MAX_LG));
     if (p->gen_row || p->gen_col)
     {
-        // lengths that are not powers of two: general kernels, FP64 / FP32
-        if (p->prec != P_F64 && p->prec != P_F32)
-            return fail(HPXFFT_UNSUPPORTED_SIZE, "lengths other than powers of two (or beyond Nx = 65536, Ny = "
-                                                     "32768) need an F64 or F32 plan (got " + dims + ")");
+        // lengths that are not powers of two: general kernels (FP64 / FP32;
+        // 16-bit storage within one workgroup, forward only)
+        const bool s16 = p->prec == P_F16 || p->prec == P_BF16;
         const size_t cap = p->prec == P_F64 ? hpxfft_dev::gen_max_points<double>()
                                             : hpxfft_dev::gen_max_points<float>();
         auto why = [&](const char *what, size_t n) {
@@ -956,6 +990,10 @@ int plan_common(hpxfft_plan_s *p)
         const size_t wmax = 128 / (2 * storage_bytes(p->prec));   // one 128-byte segment
         if (p->gen_row && !gen_setup(p->ny / 2, cap, p->grow))
         {
+            if (s16)
+                return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two only "
+                                                     "within one workgroup: F64 or F32 for row length Ny/2 = " +
+                                                     std::to_string(p->ny / 2) + " (got " + dims + ")");
             if (!gen2_setup(p->ny / 2, cap, p->g2r))
                 return fail(HPXFFT_UNSUPPORTED_SIZE, why("row length Ny/2", p->ny / 2));
             if (p->nx > 65535 || p->batch > 65535)
@@ -965,6 +1003,10 @@ int plan_common(hpxfft_plan_s *p)
         }
         if (p->gen_col && !gen_setup(p->nx, cap, p->gcol))
         {
+            if (s16)
+                return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two only "
+                                                     "within one workgroup: F64 or F32 for column length Nx = " +
+                                                     std::to_string(p->nx) + " (got " + dims + ")");
             if (!gen2_setup(p->nx, cap, p->g2c))
                 return fail(HPXFFT_UNSUPPORTED_SIZE, why("column length Nx", p->nx));
             p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
@@ -982,7 +1024,7 @@ int plan_common(hpxfft_plan_s *p)
             p->gen_w = gen_width(p->gcol, cap, wmax);
             // narrow strips (16- or 32-byte row segments): at measure effort the
             // two-level form, with whole segments, is timed against it
-            if (p->effort >= HPXFFT_MEASURE && p->gen_w <= 2 && gen2_setup(p->nx, cap, p->g2c))
+            if (!s16 && p->effort >= HPXFFT_MEASURE && p->gen_w <= 2 && gen2_setup(p->nx, cap, p->g2c))
             {
                 p->g2c.on = false;
                 p->g2c.WA = gen_width(p->g2c.A, cap, wmax);

@@ -248,6 +248,8 @@ __device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, con
     return G;
 }
 
+__device__ __forceinline__ float absmax(cpx<float> z) { return fmaxf(fabsf(z.x), fabsf(z.y)); }
+
 template <typename T> __device__ __forceinline__ cpx<T> *gen_lds()
 {
     extern __shared__ __align__(16) unsigned char gen_smem[];
@@ -259,42 +261,60 @@ template <typename T> __device__ __forceinline__ cpx<T> *gen_lds()
 //   X[k] = Fe + w_{Ny}^k Fo, X[L-k] = conj(Fe - w_{Ny}^k Fo),
 //   Fe = (Z[k] + conj Z[L-k]) / 2, Fo = (Z[k] - conj Z[L-k]) / (2i).
 // tabL: the row plan's table; tab2: w_{Ny}^m (m < Ny).  blockIdx.x = row group
-// (XCD order), .y = grid.  Dynamic LDS: 2 * nseq * g.n complex.
-template <typename T>
+// (XCD order), .y = grid.  Dynamic LDS: 2 * nseq * g.n complex, then nseq
+// words.  16-bit storage S (T = float): the input row is unscaled by
+// 2^-in_exp[row] when given, and the output row is stored times 2^e with
+// e = pick_exp(its largest component), written to out_exp[row], as the
+// power-of-two row kernels do.
+template <typename T, typename S = T>
 __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan g, const cpx<double> *tabL,
                                                             const cpx<double> *tab2, int nseq)
 {
+    constexpr bool S16 = is16<S>();
     cpx<T> *buf = gen_lds<T>();
     const int t = threadIdx.x, L = g.len, ST = g.n;
+    unsigned *smax = reinterpret_cast<unsigned *>(buf + 2 * nseq * ST);
     const long long b = blockIdx.y;
     const int row0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * nseq;
     const int nrow = min(nseq, a.nrows - row0);
-    const T *in = reinterpret_cast<const T *>(a.in) + b * a.dist_in + (long long)row0 * a.ld_in;
-    T *out = reinterpret_cast<T *>(a.out) + b * a.dist_out + (long long)row0 * a.ld_out;
+    const S *in = reinterpret_cast<const S *>(a.in) + b * a.dist_in + (long long)row0 * a.ld_in;
+    S *out = reinterpret_cast<S *>(a.out) + b * a.dist_out + (long long)row0 * a.ld_out;
+    if constexpr (S16)
+    {
+        for (int i = t; i < nseq; i += GEN_NT) smax[i] = 0u;   // ordered before use by the barriers below
+    }
     const float inv_l = 1.0f / (float)L;
     for (int i = t; i < nseq * L; i += GEN_NT)
     {
         const int s = gdiv(i, L, inv_l), e = i - s * L;
-        buf[s * ST + e] = s < nrow ? stor<T>::ld(in + s * a.ld_in + 2 * e) : cpx<T>{T(0), T(0)};
+        cpx<T> z = {T(0), T(0)};
+        if (s < nrow)
+        {
+            z = stor<S>::ld(in + s * a.ld_in + 2 * e);
+            if constexpr (S16)
+            {
+                if (a.in_exp) z = scal(z, (T)ldexpf(1.0f, -a.in_exp[b * a.exp_dist + row0 + s]));
+            }
+        }
+        buf[s * ST + e] = z;
     }
     __syncthreads();
     const cpx<T> *Zs = gen_fft<T>(buf, buf + nseq * ST, nseq, g, tabL);
     const int half = L / 2 + 1;   // k = 0 .. floor(L/2)
     const float inv_h = 1.0f / (float)half;
-    for (int i = t; i < nrow * half; i += GEN_NT)
-    {
-        const int s = gdiv(i, half, inv_h), k = i - s * half;
+    // X[k] and X[L-k] of row s (X[0] and X[L] for k = 0; X[L/2] alone for 2k = L)
+    auto split = [&](int s, int k, cpx<T> &x0, cpx<T> &x1) {
         const cpx<T> *Z = Zs + s * ST;
-        T *o = out + s * a.ld_out;
         if (k == 0)
         {
             const cpx<T> z0 = Z[0];
-            stor<T>::st(o, cpx<T>{z0.x + z0.y, T(0)});
-            stor<T>::st(o + 2 * L, cpx<T>{z0.x - z0.y, T(0)});
+            x0 = cpx<T>{z0.x + z0.y, T(0)};
+            x1 = cpx<T>{z0.x - z0.y, T(0)};
         }
         else if (2 * k == L)
         {
-            stor<T>::st(o + 2 * k, conjc(Z[k]));   // pairs with itself
+            x0 = conjc(Z[k]);
+            x1 = x0;
         }
         else
         {
@@ -303,8 +323,44 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
             const cpx<T> D = scal(A - Bc, T(0.5));
             const cpx<T> Fo = {D.y, -D.x};
             const cpx<T> tt = cmul(tabv<T>(tab2, k), Fo);
-            stor<T>::st(o + 2 * k, Fe + tt);
-            stor<T>::st(o + 2 * (L - k), conjc(Fe - tt));
+            x0 = Fe + tt;
+            x1 = conjc(Fe - tt);
+        }
+    };
+    if constexpr (S16)
+    {
+        for (int i = t; i < nrow * half; i += GEN_NT)
+        {
+            const int s = gdiv(i, half, inv_h), k = i - s * half;
+            cpx<T> x0, x1;
+            split(s, k, x0, x1);
+            atomicMax(&smax[s], __float_as_uint(fmaxf(absmax(x0), absmax(x1))));
+        }
+        __syncthreads();
+    }
+    for (int i = t; i < nrow * half; i += GEN_NT)
+    {
+        const int s = gdiv(i, half, inv_h), k = i - s * half;
+        cpx<T> x0, x1;
+        split(s, k, x0, x1);
+        S *o = out + s * a.ld_out;
+        if constexpr (S16)
+        {
+            const int ex = pick_exp(__uint_as_float(smax[s]));
+            const T sc = (T)ldexpf(1.0f, ex);
+            x0 = scal(x0, sc);
+            x1 = scal(x1, sc);
+            if (k == 0 && a.out_exp) a.out_exp[b * a.exp_dist + row0 + s] = ex;
+        }
+        if (k == 0)
+        {
+            stor<S>::st(o, x0);
+            stor<S>::st(o + 2 * L, x1);
+        }
+        else
+        {
+            stor<S>::st(o + 2 * k, x0);
+            if (2 * k != L) stor<S>::st(o + 2 * (L - k), x1);
         }
     }
 }
@@ -314,33 +370,65 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
 // forward transform, in place).  CONJ: the inverse transform, computed as
 // conj(DFT(conj .)).  blockIdx.x = strip (XCD order: neighbouring strips,
 // which share 128-byte lines, run behind one L2), .y = grid.
-// tab: the column plan's table.  Dynamic LDS: 2 * W * g.n complex.
-template <typename T, bool CONJ>
+// tab: the column plan's table.  Dynamic LDS: 2 * W * g.n complex, then W
+// words.  16-bit storage S (T = float, forward only): rows are unscaled by
+// 2^-row_exp[row] on load, and column c is stored times 2^e with e =
+// pick_exp(its largest component), written to out_exp[c].
+template <typename T, typename S, bool CONJ>
 __global__ void __launch_bounds__(GEN_NT) gen_col_kernel(ColArgs a, GenPlan g, const cpx<double> *tab, int W)
 {
+    constexpr bool S16 = is16<S>();
     cpx<T> *buf = gen_lds<T>();
     const int t = threadIdx.x, n = g.len, ST = g.n;
+    unsigned *smax = reinterpret_cast<unsigned *>(buf + 2 * W * ST);
     const long long b = blockIdx.y;
     const int col0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * W;
     const int nvalid = min(W, a.ncols - col0);
-    const T *src = reinterpret_cast<const T *>(a.in) + 2 * b * a.dist;
-    T *dst = reinterpret_cast<T *>(a.out) + 2 * b * a.dist;
+    const S *src = reinterpret_cast<const S *>(a.in) + 2 * b * a.dist;
+    S *dst = reinterpret_cast<S *>(a.out) + 2 * b * a.dist;
+    if constexpr (S16)
+    {
+        if (t < W) smax[t] = 0u;   // ordered before use by the barriers below
+    }
     const float inv_w = 1.0f / (float)W;
-    // row segments: element (e, w) -> buf[w * n + e]
+    // row segments: element (e, w) -> buf[w * ST + e]
     for (int i = t; i < n * W; i += GEN_NT)
     {
         const int e = gdiv(i, W, inv_w), w = i - e * W;
         cpx<T> z = {T(0), T(0)};
-        if (w < nvalid) z = stor<T>::ld(src + 2 * ((long long)e * a.pitch + col0 + w));
+        if (w < nvalid)
+        {
+            z = stor<S>::ld(src + 2 * ((long long)e * a.pitch + col0 + w));
+            if constexpr (S16)
+            {
+                if (a.row_exp) z = scal(z, (T)ldexpf(1.0f, -a.row_exp[b * a.rexp_dist + e]));
+            }
+        }
         buf[w * ST + e] = CONJ ? conjc(z) : z;
     }
     __syncthreads();
     const cpx<T> *X = gen_fft<T>(buf, buf + ST * W, W, g, tab);
+    if constexpr (S16)
+    {
+        for (int i = t; i < n * W; i += GEN_NT)
+        {
+            const int e = gdiv(i, W, inv_w), w = i - e * W;
+            if (w < nvalid) atomicMax(&smax[w], __float_as_uint(absmax(X[w * ST + e])));
+        }
+        __syncthreads();
+    }
     for (int i = t; i < n * W; i += GEN_NT)
     {
         const int e = gdiv(i, W, inv_w), w = i - e * W;
-        const cpx<T> z = X[w * ST + e];
-        if (w < nvalid) stor<T>::st(dst + 2 * ((long long)e * a.pitch + col0 + w), CONJ ? conjc(z) : z);
+        if (w >= nvalid) continue;
+        cpx<T> z = X[w * ST + e];
+        if constexpr (S16)
+        {
+            const int ex = pick_exp(__uint_as_float(smax[w]));
+            z = scal(z, (T)ldexpf(1.0f, ex));
+            if (e == 0 && a.out_exp) a.out_exp[b * a.cexp_dist + col0 + w] = ex;
+        }
+        stor<S>::st(dst + 2 * ((long long)e * a.pitch + col0 + w), CONJ ? conjc(z) : z);
     }
 }
 

@@ -42,7 +42,8 @@ typedef enum
                                     lengths (Nx, Ny/2) have prime factors <= 61 within 4096 (F64) /
                                     8192 (F32) points, or otherwise up to 2048 (F64) / 4096 (F32)
                                     points by Bluestein's algorithm, or up to 2^20 points as a
-                                    product of two such lengths */
+                                    product of two such lengths (F16/BF16 storage: the one-level
+                                    lengths, forward) */
     HPXFFT_NO_DEVICE = 3,        /* no HIP device visible */
     HPXFFT_OUT_OF_MEMORY = 4,
     HPXFFT_HIP_ERROR = 5,

@@ -94,9 +94,18 @@ def test_fp32_bluestein_reaches_twice_as_far():
 
 
 @pytest.mark.parametrize("prec", [2, 3])
-def test_general_lengths_need_f64_or_f32(prec):
-    assert plan_status(8, 14, prec=prec) == _lib.UNSUPPORTED_SIZE
-    assert "F64 or F32" in hpxfft_amd.last_error()
+def test_general_lengths_16bit_within_one_workgroup(prec):
+    """16-bit storage takes lengths that are not powers of two within one
+    workgroup (here 8 x 14 and a Bluestein length); lengths that need two
+    levels ask for an F64 or F32 plan."""
+    for nx, ny in [(8, 14), (67, 16), (16, 134)]:
+        st = plan_status(nx, ny, prec=prec)
+        if st == _lib.OK:
+            pytest.skip("a HIP device is present")
+        assert st == _lib.NO_DEVICE, hpxfft_amd.last_error()
+    for nx, ny in [(10000, 8), (8, 2 * 10000)]:
+        assert plan_status(nx, ny, prec=prec) == _lib.UNSUPPORTED_SIZE
+        assert "F64 or F32" in hpxfft_amd.last_error() and "16-bit" in hpxfft_amd.last_error()
 
 
 @pytest.mark.parametrize("nx,ny", [(8, 14), (3, 16), (1000, 64), (64, 1000), (4095, 2),

@@ -40,7 +40,10 @@ def to16(x_pad, ny, dt, scale_rows):
 @pytest.mark.parametrize("nx,ny,amp", [(8, 16, 1.0), (64, 256, 1.0), (256, 1024, 1e6), (1024, 512, 1.0),
                                        (4096, 64, 1e6), (512, 8192, 1.0), (64, 32768, 1.0),
                                        (32768, 64, 1e6), (256, 4096, 1e6), (128, 16384, 1.0),
-                                       (32768, 256, 1.0)])
+                                       (32768, 256, 1.0),
+                                       # lengths that are not powers of two (general kernels)
+                                       (8, 14, 1.0), (100, 134, 1e6), (67, 256, 1.0), (3000, 64, 1e6),
+                                       (64, 3000, 1.0), (1000, 1000, 1e6), (6000, 16, 1.0), (4096, 30, 1e6)])
 def test_16bit_storage(gpu, dt_name, bound, nx, ny, amp):
     import torch
     import hpxfft_amd
