@@ -375,6 +375,12 @@ def main() -> int:
                      "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                      "alg_bytes_per_launch": dom_bytes, "avg_launch_s": dom_s},
         "launches_s": {k: v[1] for k, v in per_launch.items()},
+        # every launch: algorithmic bytes, HBM traffic (committed counter summary) and both rates
+        "launches": {k: {"s": v[1], "alg_bytes": v[0], "alg_GBps": round(v[0] / v[1] / 1e9, 1),
+                         "traffic": measured_traffic(workload).get(k),
+                         "traffic_GBps": (round(measured_traffic(workload)[k] / v[1] / 1e9, 1)
+                                          if measured_traffic(workload).get(k) else None)}
+                     for k, v in per_launch.items()},
         "plan": {"effort": "measure", "seconds": round(r64["plan_s"], 4),
                  "column_pass": [ln.strip() for ln in r64["plan"].splitlines() if "four-step" in ln or "autotuned" in ln]},
         "phases_s": {"row_r2c": r64["row_s"], "column_c2c": r64["col_s"]},
