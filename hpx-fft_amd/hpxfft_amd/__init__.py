@@ -14,7 +14,8 @@ __all__ = ["HpxfftError", "lib", "last_error", "LIB_PATH", "Plan", "BatchedPlan"
 
 def __getattr__(name):
     # torch-dependent entry points are imported lazily
-    if name in ("rfft2", "rfft2_adjoint", "c2r_2d", "rfft2_scaled", "RFFT2", "rfft2_autograd", "clear_plan_cache"):
+    if name in ("rfft2", "rfft2_adjoint", "c2r_2d", "rfft2_scaled", "RFFT2", "rfft2_autograd", "clear_plan_cache",
+                "set_planning_effort", "planning_effort"):
         from . import torch_api
         return getattr(torch_api, name)
     raise AttributeError(name)

@@ -8,6 +8,10 @@ current HIP stream; all arithmetic runs in libhpxfft_hip.so).
                               c2r_2d(rfft2(x), Ny) = Nx*Ny*x.
 * :class:`RFFT2`           -- torch.autograd.Function: forward = rfft2,
                               backward = rfft2_adjoint (hand-written kernels).
+* :func:`set_planning_effort` -- the planning effort of the cached plans these
+                              functions create ("estimate" by default;
+                              "measure" times the column-pass choices once
+                              per shape, as FFTW's MEASURE does).
 """
 from __future__ import annotations
 
@@ -17,16 +21,31 @@ from .plan import Plan
 
 _DT = {torch.float64: ("f64", torch.complex128), torch.float32: ("f32", torch.complex64)}
 _plans: dict = {}
+_EFFORTS = ("estimate", "measure", "patient", "exhaustive")
+_effort = "estimate"
+
+
+def set_planning_effort(effort: str) -> None:
+    """Planning effort of the plans created from now on (plans already
+    cached keep theirs; clear_plan_cache() drops them)."""
+    global _effort
+    if effort not in _EFFORTS:
+        raise ValueError(f"Invalid FFTW plan flag string: {effort!r}")
+    _effort = effort
+
+
+def planning_effort() -> str:
+    return _effort
 
 
 def _get_plan(nx: int, ny: int, batch: int, precision: str, ld_in: int, ld_out: int, device: torch.device) -> Plan:
     stream = torch.cuda.current_stream(device)
-    key = (nx, ny, batch, precision, ld_in, ld_out, device.index, stream.cuda_stream)
+    key = (nx, ny, batch, precision, ld_in, ld_out, device.index, stream.cuda_stream, _effort)
     p = _plans.get(key)
     if p is None:
         with torch.cuda.device(device):
             p = Plan(nx, ny, batch=batch, ld_in=ld_in, dist_in=nx * ld_in, ld_out=ld_out, dist_out=nx * ld_out,
-                     precision=precision, stream=stream)
+                     precision=precision, stream=stream, effort=_effort)
         _plans[key] = p
     return p
 

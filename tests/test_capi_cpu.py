@@ -273,3 +273,15 @@ def test_batched_plan_argument_errors():
     assert L.hpxfft_plan_r2c_2d_batched(ctypes.byref(h), 0, arr(64, 64, 64), arr(64, 64, 64), 1, 0,
                                         None) == _lib.INVALID_ARG
     assert L.hpxfft_exec_r2c_2d_batched(None, None) == _lib.INVALID_ARG
+
+
+def test_torch_planning_effort_setting():
+    """The torch entry points' planning effort: the reference's flag strings
+    only (adapter_fftw.hpp:22-44), estimate by default."""
+    import hpxfft_amd
+    assert hpxfft_amd.planning_effort() == "estimate"
+    with pytest.raises(ValueError, match="Invalid FFTW plan flag string"):
+        hpxfft_amd.set_planning_effort("quick")
+    hpxfft_amd.set_planning_effort("measure")
+    assert hpxfft_amd.planning_effort() == "measure"
+    hpxfft_amd.set_planning_effort("estimate")

@@ -184,3 +184,26 @@ def test_fp32_gradient_batch16_4096(gpu):
     g = torch.Generator(device=gpu).manual_seed(5)
     x = torch.randn(16, 4096, 4096, dtype=torch.float32, device=gpu, generator=g)
     assert directional_check(x, ndir=2) <= 1e-6
+
+
+def test_torch_planning_effort_measure(gpu):
+    """Plans made at "measure" through the torch entry points (the column
+    split is timed) give the oracle's results; the setting is validated."""
+    import torch
+    import hpxfft_amd
+    with pytest.raises(ValueError, match="Invalid FFTW plan flag string"):
+        hpxfft_amd.set_planning_effort("fast")
+    nx, ny = 4096, 256
+    rng = np.random.default_rng(17)
+    Y = rng.standard_normal((nx, ny // 2 + 1)) + 1j * rng.standard_normal((nx, ny // 2 + 1))
+    x = torch.randn(nx, ny, dtype=torch.float64, device=gpu)
+    hpxfft_amd.set_planning_effort("measure")
+    try:
+        X = hpxfft_amd.rfft2(x)
+        adj = hpxfft_amd.rfft2_adjoint(torch.from_numpy(Y).to(gpu), ny)
+        assert hpxfft_amd.planning_effort() == "measure"
+    finally:
+        hpxfft_amd.set_planning_effort("estimate")
+        hpxfft_amd.clear_plan_cache()
+    assert oracle.rel_l2(X.cpu().numpy(), oracle.r2c_2d(x.cpu().numpy())) <= 1e-12
+    assert oracle.rel_l2(adj.cpu().numpy(), oracle.adjoint_r2c_2d(Y, ny)) <= 1e-12
