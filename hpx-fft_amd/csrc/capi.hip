@@ -23,6 +23,21 @@
 
 using namespace hpxfft_host;
 
+// launch a general kernel built for the launch bound that covers `nt`
+// threads (256, or 1024 for the larger workgroups); TARGS are its other
+// template arguments in parentheses
+#define GEN_UNPAREN(...) __VA_ARGS__
+#define GEN_LAUNCH(KERN, TARGS, nt, grid, lds, stream, ...)                                                        \
+    do                                                                                                             \
+    {                                                                                                              \
+        if ((nt) <= (unsigned)hpxfft_dev::GEN_NT)                                                                  \
+            hipLaunchKernelGGL((KERN<GEN_UNPAREN TARGS, hpxfft_dev::GEN_NT>), grid, dim3(nt), lds, stream,         \
+                               __VA_ARGS__);                                                                       \
+        else                                                                                                       \
+            hipLaunchKernelGGL((KERN<GEN_UNPAREN TARGS, hpxfft_dev::GEN_NT_MAX>), grid, dim3(nt), lds, stream,     \
+                               __VA_ARGS__);                                                                       \
+    } while (0)
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -260,6 +275,25 @@ size_t gen_lds_bytes(const hpxfft_plan_s *p, size_t points, size_t words = 0)
     return 2 * points * 2 * compute_bytes(p->prec) + 4 * words;
 }
 
+// threads of a general workgroup whose dynamic LDS is `lds` bytes: 256 while
+// three or more workgroups fit a CU's 160 KiB; a block that leaves room for
+// only two (one) workgroups per CU takes 512 (1024) threads, so the CU still
+// has 16-32 waves to hide latency with (measured: 3000 x 3000 FP64 columns,
+// one column of 96 KiB per workgroup, 134 -> 116 us; rows of 48 KiB stay
+// fastest at 256)
+unsigned gen_threads(size_t lds, const hpxfft_dev::GenPlan &g)
+{
+    constexpr size_t CU_LDS = 160 * 1024;
+    const size_t per_cu = CU_LDS / std::max<size_t>(lds, 1);
+    // a prime radix above 7 runs the generic stage, whose outputs each loop
+    // over r inputs: twice the threads hide that loop's latency (measured:
+    // rows of 516 = 4*3*43 points, 46 -> 34 us at 512 threads)
+    bool generic = false;
+    for (int i = 0; i < g.nstages; ++i) generic = generic || g.radix[i] > 7;
+    if (per_cu >= 3) return generic ? 512u : 256u;
+    return per_cu == 2 ? 512u : 1024u;
+}
+
 // let every general kernel take up to GEN_LDS_BYTES of dynamic LDS (above the
 // 64 KiB default); once per device
 int gen_allow_lds(int dev)
@@ -269,19 +303,18 @@ int gen_allow_lds(int dev)
     std::lock_guard<std::mutex> lk(mu);
     if (done[dev]) return HPXFFT_OK;
     using namespace hpxfft_dev;
+#define GEN_BOTH(KERN, ...) reinterpret_cast<const void *>(&KERN<__VA_ARGS__, GEN_NT>), \
+                            reinterpret_cast<const void *>(&KERN<__VA_ARGS__, GEN_NT_MAX>)
     const void *fns[] = {
-        reinterpret_cast<const void *>(&gen_row_r2c_kernel<double>), reinterpret_cast<const void *>(&gen_row_r2c_kernel<float>),
-        reinterpret_cast<const void *>(&gen_row_r2c_kernel<float, __half>),
-        reinterpret_cast<const void *>(&gen_row_r2c_kernel<float, bf16_t>),
-        reinterpret_cast<const void *>(&gen_col_kernel<double, double, false>),
-        reinterpret_cast<const void *>(&gen_col_kernel<double, double, true>),
-        reinterpret_cast<const void *>(&gen_col_kernel<float, float, false>),
-        reinterpret_cast<const void *>(&gen_col_kernel<float, float, true>),
-        reinterpret_cast<const void *>(&gen_col_kernel<float, __half, false>),
-        reinterpret_cast<const void *>(&gen_col_kernel<float, bf16_t, false>),
-        reinterpret_cast<const void *>(&gen_row_c2r_kernel<double, false>), reinterpret_cast<const void *>(&gen_row_c2r_kernel<double, true>),
-        reinterpret_cast<const void *>(&gen_row_c2r_kernel<float, false>), reinterpret_cast<const void *>(&gen_row_c2r_kernel<float, true>),
-        reinterpret_cast<const void *>(&gen_strided_kernel<double>), reinterpret_cast<const void *>(&gen_strided_kernel<float>)};
+        GEN_BOTH(gen_row_r2c_kernel, double, double), GEN_BOTH(gen_row_r2c_kernel, float, float),
+        GEN_BOTH(gen_row_r2c_kernel, float, __half), GEN_BOTH(gen_row_r2c_kernel, float, bf16_t),
+        GEN_BOTH(gen_col_kernel, double, double, false), GEN_BOTH(gen_col_kernel, double, double, true),
+        GEN_BOTH(gen_col_kernel, float, float, false), GEN_BOTH(gen_col_kernel, float, float, true),
+        GEN_BOTH(gen_col_kernel, float, __half, false), GEN_BOTH(gen_col_kernel, float, bf16_t, false),
+        GEN_BOTH(gen_row_c2r_kernel, double, false), GEN_BOTH(gen_row_c2r_kernel, double, true),
+        GEN_BOTH(gen_row_c2r_kernel, float, false), GEN_BOTH(gen_row_c2r_kernel, float, true),
+        GEN_BOTH(gen_strided_kernel, double), GEN_BOTH(gen_strided_kernel, float)};
+#undef GEN_BOTH
     // the buffers plus up to 1024 words of per-row / per-column maxima
     for (const void *f : fns)
         HIPCHK(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, GEN_LDS_BYTES + 4096));
@@ -508,7 +541,9 @@ std::string describe(const hpxfft_plan_s *p)
     }
     else if (p->gen_row)
         os << "  row r2c kernel (general length): " << p->ny / 2 << "-point " << gen_method(p->grow) << ", "
-           << p->gen_rows << " row(s)/workgroup, " << hpxfft_dev::GEN_NT << " threads + r2c split\n";
+           << p->gen_rows << " row(s)/workgroup, "
+           << gen_threads(gen_lds_bytes(p, (size_t)p->gen_rows * p->grow.n, (size_t)p->gen_rows), p->grow)
+           << " threads + r2c split\n";
     else
     {
         const geom rg = row_geom(p->prec, p->lgL);
@@ -577,13 +612,14 @@ int launch_strided(hpxfft_plan_s *p, const hpxfft_dev::GenStride &a, const hpxff
 {
     const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
     const size_t lds = gen_lds_bytes(p, (size_t)W * g.n);
-    const dim3 grid(gx, gy, gz), block(hpxfft_dev::GEN_NT);
+    const dim3 grid(gx, gy, gz);
+    const unsigned nt = gen_threads(lds, g);
     if (p->prec == P_F64)
-        hipLaunchKernelGGL(hpxfft_dev::gen_strided_kernel<double>, grid, block, lds, p->stream, a, g, tab + tab_off,
-                           tab + twt_off, tab + p->gtab_2, W);
+        GEN_LAUNCH(hpxfft_dev::gen_strided_kernel, (double), nt, grid, lds, p->stream, a, g, tab + tab_off,
+                   tab + twt_off, tab + p->gtab_2, W);
     else
-        hipLaunchKernelGGL(hpxfft_dev::gen_strided_kernel<float>, grid, block, lds, p->stream, a, g, tab + tab_off,
-                           tab + twt_off, tab + p->gtab_2, W);
+        GEN_LAUNCH(hpxfft_dev::gen_strided_kernel, (float), nt, grid, lds, p->stream, a, g, tab + tab_off,
+                   tab + twt_off, tab + p->gtab_2, W);
     HIPCHK(hipGetLastError());
     return HPXFFT_OK;
 }
@@ -726,8 +762,8 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         a.in = spec_in;
         a.out = spec_out;
         const dim3 grid((unsigned)((p->ncols + p->gen_w - 1) / p->gen_w), (unsigned)p->batch);
-        const dim3 block(hpxfft_dev::GEN_NT);
         const size_t lds = gen_lds_bytes(p, (size_t)p->gen_w * p->gcol.n, (size_t)p->gen_w);
+        const unsigned nt = gen_threads(lds, p->gcol);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab) + p->gtab_N;
         using namespace hpxfft_dev;
         const GenPlan &g = p->gcol;
@@ -735,20 +771,20 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         switch (p->prec)
         {
         case P_F64:
-            if (conj) hipLaunchKernelGGL((gen_col_kernel<double, double, true>), grid, block, lds, p->stream, a, g, tab, W);
-            else hipLaunchKernelGGL((gen_col_kernel<double, double, false>), grid, block, lds, p->stream, a, g, tab, W);
+            if (conj) GEN_LAUNCH(gen_col_kernel, (double, double, true), nt, grid, lds, p->stream, a, g, tab, W);
+            else GEN_LAUNCH(gen_col_kernel, (double, double, false), nt, grid, lds, p->stream, a, g, tab, W);
             break;
         case P_F32:
-            if (conj) hipLaunchKernelGGL((gen_col_kernel<float, float, true>), grid, block, lds, p->stream, a, g, tab, W);
-            else hipLaunchKernelGGL((gen_col_kernel<float, float, false>), grid, block, lds, p->stream, a, g, tab, W);
+            if (conj) GEN_LAUNCH(gen_col_kernel, (float, float, true), nt, grid, lds, p->stream, a, g, tab, W);
+            else GEN_LAUNCH(gen_col_kernel, (float, float, false), nt, grid, lds, p->stream, a, g, tab, W);
             break;
         case P_F16:
             if (conj) return fail(HPXFFT_INVALID_ARG, "16-bit storage plans are forward only");
-            hipLaunchKernelGGL((gen_col_kernel<float, __half, false>), grid, block, lds, p->stream, a, g, tab, W);
+            GEN_LAUNCH(gen_col_kernel, (float, __half, false), nt, grid, lds, p->stream, a, g, tab, W);
             break;
         default:
             if (conj) return fail(HPXFFT_INVALID_ARG, "16-bit storage plans are forward only");
-            hipLaunchKernelGGL((gen_col_kernel<float, bf16_t, false>), grid, block, lds, p->stream, a, g, tab, W);
+            GEN_LAUNCH(gen_col_kernel, (float, bf16_t, false), nt, grid, lds, p->stream, a, g, tab, W);
             break;
         }
         HIPCHK(hipGetLastError());
@@ -832,25 +868,18 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     else if (p->gen_row)
     {
         const int ns = p->gen_rows;
-        const dim3 grid((unsigned)((p->nx + ns - 1) / ns), (unsigned)p->batch), block(hpxfft_dev::GEN_NT);
         const size_t lds = gen_lds_bytes(p, (size_t)ns * p->grow.n, (size_t)ns);
+        const dim3 grid((unsigned)((p->nx + ns - 1) / ns), (unsigned)p->batch);
+        const unsigned nt = gen_threads(lds, p->grow);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
         const auto *tL = tab + p->gtab_L, *t2 = tab + p->gtab_2;
         using namespace hpxfft_dev;
         switch (p->prec)
         {
-        case P_F64:
-            hipLaunchKernelGGL((gen_row_r2c_kernel<double, double>), grid, block, lds, p->stream, r, p->grow, tL, t2, ns);
-            break;
-        case P_F32:
-            hipLaunchKernelGGL((gen_row_r2c_kernel<float, float>), grid, block, lds, p->stream, r, p->grow, tL, t2, ns);
-            break;
-        case P_F16:
-            hipLaunchKernelGGL((gen_row_r2c_kernel<float, __half>), grid, block, lds, p->stream, r, p->grow, tL, t2, ns);
-            break;
-        default:
-            hipLaunchKernelGGL((gen_row_r2c_kernel<float, bf16_t>), grid, block, lds, p->stream, r, p->grow, tL, t2, ns);
-            break;
+        case P_F64: GEN_LAUNCH(gen_row_r2c_kernel, (double, double), nt, grid, lds, p->stream, r, p->grow, tL, t2, ns); break;
+        case P_F32: GEN_LAUNCH(gen_row_r2c_kernel, (float, float), nt, grid, lds, p->stream, r, p->grow, tL, t2, ns); break;
+        case P_F16: GEN_LAUNCH(gen_row_r2c_kernel, (float, __half), nt, grid, lds, p->stream, r, p->grow, tL, t2, ns); break;
+        default: GEN_LAUNCH(gen_row_r2c_kernel, (float, bf16_t), nt, grid, lds, p->stream, r, p->grow, tL, t2, ns); break;
         }
         HIPCHK(hipGetLastError());
     }
@@ -1338,19 +1367,22 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     else if (plan->gen_row)
     {
         const int ns = plan->gen_rows;
-        const dim3 grid((unsigned)((plan->nx + ns - 1) / ns), (unsigned)plan->batch), block(hpxfft_dev::GEN_NT);
         const size_t lds = gen_lds_bytes(plan, (size_t)ns * plan->grow.n);
+        const dim3 grid((unsigned)((plan->nx + ns - 1) / ns), (unsigned)plan->batch);
+        const unsigned nt = gen_threads(lds, plan->grow);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(plan->gen_tab);
         const auto *tL = tab + plan->gtab_L, *t2 = tab + plan->gtab_2;
+        using namespace hpxfft_dev;
+        const hipStream_t st = plan->stream;
         if (plan->prec == P_F64)
         {
-            if (halve) hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<double, true>), grid, block, lds, plan->stream, r, plan->grow, tL, t2, ns);
-            else hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<double, false>), grid, block, lds, plan->stream, r, plan->grow, tL, t2, ns);
+            if (halve) GEN_LAUNCH(gen_row_c2r_kernel, (double, true), nt, grid, lds, st, r, plan->grow, tL, t2, ns);
+            else GEN_LAUNCH(gen_row_c2r_kernel, (double, false), nt, grid, lds, st, r, plan->grow, tL, t2, ns);
         }
         else
         {
-            if (halve) hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<float, true>), grid, block, lds, plan->stream, r, plan->grow, tL, t2, ns);
-            else hipLaunchKernelGGL((hpxfft_dev::gen_row_c2r_kernel<float, false>), grid, block, lds, plan->stream, r, plan->grow, tL, t2, ns);
+            if (halve) GEN_LAUNCH(gen_row_c2r_kernel, (float, true), nt, grid, lds, st, r, plan->grow, tL, t2, ns);
+            else GEN_LAUNCH(gen_row_c2r_kernel, (float, false), nt, grid, lds, st, r, plan->grow, tL, t2, ns);
         }
         HIPCHK(hipGetLastError());
     }

@@ -25,9 +25,17 @@ namespace hpxfft_dev {
 
 constexpr int GEN_MAX_STAGES = 24;
 constexpr int GEN_MAX_RADIX = 61;
-constexpr int GEN_NT = 256;                  // threads per workgroup
+// threads per workgroup: 256, or up to GEN_NT_MAX for large LDS blocks (each
+// kernel is built for both bounds, NTMAX = 256 and 1024; loops stride by the
+// launch's blockDim)
+constexpr int GEN_NT = 256;
+constexpr int GEN_NT_MAX = 1024;
 constexpr int GEN_LDS_BYTES = 128 * 1024;    // the largest pair of sequence buffers of a workgroup
 template <typename T> constexpr int gen_max_points() { return GEN_LDS_BYTES / (2 * 2 * (int)sizeof(T)); }
+
+// loop stride of a workgroup: the 256-thread build knows it at compile time,
+// the larger build reads it from the launch (512 or 1024)
+template <int NTMAX> __device__ __forceinline__ int gen_nth() { return NTMAX == GEN_NT ? GEN_NT : (int)blockDim.x; }
 
 struct GenPlan
 {
@@ -94,7 +102,7 @@ template <typename T, int R> __device__ __forceinline__ void dft_small(cpx<T> *x
 }
 
 // one Stockham stage of radix R (2, 3, 4, 5, 7) over nseq sequences, A -> B
-template <typename T, int R>
+template <typename T, int R, int NTMAX>
 __device__ void gen_stage_reg(const cpx<T> *A, cpx<T> *B, int nseq, int n, int Ns, const cpx<double> *__restrict__ tab)
 {
     const int t = threadIdx.x, m = n / R, nb = nseq * m, tstep = n / (R * Ns);
@@ -105,7 +113,8 @@ __device__ void gen_stage_reg(const cpx<T> *A, cpx<T> *B, int nseq, int n, int N
     w[0] = cpx<T>{T(1), T(0)};
     // two butterflies per iteration (independent loads in flight together);
     // twiddles w^{qk} as powers of the one table value w^k, in FP64
-    for (int b0 = t; b0 < nb; b0 += 2 * GEN_NT)
+    const int NTH = gen_nth<NTMAX>();
+    for (int b0 = t; b0 < nb; b0 += 2 * NTH)
     {
         cpx<T> x[2][R];
         cpx<double> w1[2];
@@ -113,7 +122,7 @@ __device__ void gen_stage_reg(const cpx<T> *A, cpx<T> *B, int nseq, int n, int N
 #pragma unroll
         for (int u = 0; u < 2; ++u)
         {
-            const int b = b0 + u * GEN_NT;
+            const int b = b0 + u * NTH;
             if (b < nb)
             {
                 const int seq = gdiv(b, m, inv_m), j = b - seq * m;
@@ -152,12 +161,12 @@ __device__ void gen_stage_reg(const cpx<T> *A, cpx<T> *B, int nseq, int n, int N
 
 // one stage of a prime radix r > 7: twiddle A in place, then every output
 // reads the r inputs of its butterfly from A and goes to B
-template <typename T>
+template <typename T, int NTMAX>
 __device__ void gen_stage_any(cpx<T> *A, cpx<T> *B, int nseq, int n, int r, int Ns, const cpx<double> *__restrict__ tab)
 {
     const int t = threadIdx.x, m = n / r, total = nseq * n, tstep = n / (r * Ns);
     const float inv_n = 1.0f / (float)n, inv_m = 1.0f / (float)m, inv_ns = 1.0f / (float)Ns;
-    for (int i = t; i < total; i += GEN_NT)
+    for (int i = t; i < total; i += gen_nth<NTMAX>())
     {
         const int seq = gdiv(i, n, inv_n), idx = i - seq * n;
         const int q = gdiv(idx, m, inv_m), j = idx - q * m;
@@ -165,7 +174,7 @@ __device__ void gen_stage_any(cpx<T> *A, cpx<T> *B, int nseq, int n, int r, int 
         if (q > 0 && k > 0) A[i] = cmul(A[i], tabv<T>(tab, q * k * tstep));
     }
     __syncthreads();
-    for (int o = t; o < total; o += GEN_NT)
+    for (int o = t; o < total; o += gen_nth<NTMAX>())
     {
         const int seq = gdiv(o, n, inv_n), oo = o - seq * n;
         const int qo = gdiv(oo, m, inv_m), j = oo - qo * m;
@@ -186,7 +195,7 @@ __device__ void gen_stage_any(cpx<T> *A, cpx<T> *B, int nseq, int n, int r, int 
 
 // all stages of the plan on nseq sequences in A (B: the second buffer);
 // returns the buffer holding the result
-template <typename T>
+template <typename T, int NTMAX>
 __device__ cpx<T> *gen_stages(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
 {
     int Ns = 1;
@@ -195,12 +204,12 @@ __device__ cpx<T> *gen_stages(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, 
         const int r = g.radix[s];
         switch (r)
         {
-        case 2: gen_stage_reg<T, 2>(A, B, nseq, g.n, Ns, tab); break;
-        case 3: gen_stage_reg<T, 3>(A, B, nseq, g.n, Ns, tab); break;
-        case 4: gen_stage_reg<T, 4>(A, B, nseq, g.n, Ns, tab); break;
-        case 5: gen_stage_reg<T, 5>(A, B, nseq, g.n, Ns, tab); break;
-        case 7: gen_stage_reg<T, 7>(A, B, nseq, g.n, Ns, tab); break;
-        default: gen_stage_any<T>(A, B, nseq, g.n, r, Ns, tab); break;
+        case 2: gen_stage_reg<T, 2, NTMAX>(A, B, nseq, g.n, Ns, tab); break;
+        case 3: gen_stage_reg<T, 3, NTMAX>(A, B, nseq, g.n, Ns, tab); break;
+        case 4: gen_stage_reg<T, 4, NTMAX>(A, B, nseq, g.n, Ns, tab); break;
+        case 5: gen_stage_reg<T, 5, NTMAX>(A, B, nseq, g.n, Ns, tab); break;
+        case 7: gen_stage_reg<T, 7, NTMAX>(A, B, nseq, g.n, Ns, tab); break;
+        default: gen_stage_any<T, NTMAX>(A, B, nseq, g.n, r, Ns, tab); break;
         }
         cpx<T> *tmp = A;
         A = B;
@@ -218,28 +227,28 @@ __device__ cpx<T> *gen_stages(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, 
 // IDFT(DFT(a) * Bh) with Bh = DFT(b) / n (tab + off_b, b[m] = conj c[|m|]);
 // the inverse DFT is conj(DFT(conj .)).  Returns the buffer with X at
 // [s*g.n + k], k < g.len.
-template <typename T>
+template <typename T, int NTMAX>
 __device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
 {
-    if (g.len == g.n) return gen_stages<T>(A, B, nseq, g, tab);
+    if (g.len == g.n) return gen_stages<T, NTMAX>(A, B, nseq, g, tab);
     const int t = threadIdx.x, n = g.n, len = g.len, total = nseq * n;
     const cpx<double> *chirp = tab + g.off_c, *bh = tab + g.off_b;
     const float inv_n = 1.0f / (float)n;
-    for (int i = t; i < total; i += GEN_NT)
+    for (int i = t; i < total; i += gen_nth<NTMAX>())
     {
         const int e = i - gdiv(i, n, inv_n) * n;
         A[i] = e < len ? cmul(A[i], tabv<T>(chirp, e)) : cpx<T>{T(0), T(0)};
     }
     __syncthreads();
-    cpx<T> *F = gen_stages<T>(A, B, nseq, g, tab);
-    for (int i = t; i < total; i += GEN_NT)
+    cpx<T> *F = gen_stages<T, NTMAX>(A, B, nseq, g, tab);
+    for (int i = t; i < total; i += gen_nth<NTMAX>())
     {
         const int e = i - gdiv(i, n, inv_n) * n;
         F[i] = conjc(cmul(F[i], tabv<T>(bh, e)));
     }
     __syncthreads();
-    cpx<T> *G = gen_stages<T>(F, F == A ? B : A, nseq, g, tab);
-    for (int i = t; i < total; i += GEN_NT)
+    cpx<T> *G = gen_stages<T, NTMAX>(F, F == A ? B : A, nseq, g, tab);
+    for (int i = t; i < total; i += gen_nth<NTMAX>())
     {
         const int e = i - gdiv(i, n, inv_n) * n;
         if (e < len) G[i] = cmul(conjc(G[i]), tabv<T>(chirp, e));
@@ -266,8 +275,8 @@ template <typename T> __device__ __forceinline__ cpx<T> *gen_lds()
 // 2^-in_exp[row] when given, and the output row is stored times 2^e with
 // e = pick_exp(its largest component), written to out_exp[row], as the
 // power-of-two row kernels do.
-template <typename T, typename S = T>
-__global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan g, const cpx<double> *tabL,
+template <typename T, typename S = T, int NTMAX = GEN_NT>
+__global__ void __launch_bounds__(NTMAX) gen_row_r2c_kernel(RowArgs a, GenPlan g, const cpx<double> *tabL,
                                                             const cpx<double> *tab2, int nseq)
 {
     constexpr bool S16 = is16<S>();
@@ -281,10 +290,10 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
     S *out = reinterpret_cast<S *>(a.out) + b * a.dist_out + (long long)row0 * a.ld_out;
     if constexpr (S16)
     {
-        for (int i = t; i < nseq; i += GEN_NT) smax[i] = 0u;   // ordered before use by the barriers below
+        for (int i = t; i < nseq; i += gen_nth<NTMAX>()) smax[i] = 0u;   // ordered before use by the barriers below
     }
     const float inv_l = 1.0f / (float)L;
-    for (int i = t; i < nseq * L; i += GEN_NT)
+    for (int i = t; i < nseq * L; i += gen_nth<NTMAX>())
     {
         const int s = gdiv(i, L, inv_l), e = i - s * L;
         cpx<T> z = {T(0), T(0)};
@@ -299,7 +308,7 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
         buf[s * ST + e] = z;
     }
     __syncthreads();
-    const cpx<T> *Zs = gen_fft<T>(buf, buf + nseq * ST, nseq, g, tabL);
+    const cpx<T> *Zs = gen_fft<T, NTMAX>(buf, buf + nseq * ST, nseq, g, tabL);
     const int half = L / 2 + 1;   // k = 0 .. floor(L/2)
     const float inv_h = 1.0f / (float)half;
     // X[k] and X[L-k] of row s (X[0] and X[L] for k = 0; X[L/2] alone for 2k = L)
@@ -329,7 +338,7 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
     };
     if constexpr (S16)
     {
-        for (int i = t; i < nrow * half; i += GEN_NT)
+        for (int i = t; i < nrow * half; i += gen_nth<NTMAX>())
         {
             const int s = gdiv(i, half, inv_h), k = i - s * half;
             cpx<T> x0, x1;
@@ -338,7 +347,7 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
         }
         __syncthreads();
     }
-    for (int i = t; i < nrow * half; i += GEN_NT)
+    for (int i = t; i < nrow * half; i += gen_nth<NTMAX>())
     {
         const int s = gdiv(i, half, inv_h), k = i - s * half;
         cpx<T> x0, x1;
@@ -374,8 +383,8 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_r2c_kernel(RowArgs a, GenPlan 
 // words.  16-bit storage S (T = float, forward only): rows are unscaled by
 // 2^-row_exp[row] on load, and column c is stored times 2^e with e =
 // pick_exp(its largest component), written to out_exp[c].
-template <typename T, typename S, bool CONJ>
-__global__ void __launch_bounds__(GEN_NT) gen_col_kernel(ColArgs a, GenPlan g, const cpx<double> *tab, int W)
+template <typename T, typename S, bool CONJ, int NTMAX = GEN_NT>
+__global__ void __launch_bounds__(NTMAX) gen_col_kernel(ColArgs a, GenPlan g, const cpx<double> *tab, int W)
 {
     constexpr bool S16 = is16<S>();
     cpx<T> *buf = gen_lds<T>();
@@ -392,7 +401,7 @@ __global__ void __launch_bounds__(GEN_NT) gen_col_kernel(ColArgs a, GenPlan g, c
     }
     const float inv_w = 1.0f / (float)W;
     // row segments: element (e, w) -> buf[w * ST + e]
-    for (int i = t; i < n * W; i += GEN_NT)
+    for (int i = t; i < n * W; i += gen_nth<NTMAX>())
     {
         const int e = gdiv(i, W, inv_w), w = i - e * W;
         cpx<T> z = {T(0), T(0)};
@@ -407,17 +416,17 @@ __global__ void __launch_bounds__(GEN_NT) gen_col_kernel(ColArgs a, GenPlan g, c
         buf[w * ST + e] = CONJ ? conjc(z) : z;
     }
     __syncthreads();
-    const cpx<T> *X = gen_fft<T>(buf, buf + ST * W, W, g, tab);
+    const cpx<T> *X = gen_fft<T, NTMAX>(buf, buf + ST * W, W, g, tab);
     if constexpr (S16)
     {
-        for (int i = t; i < n * W; i += GEN_NT)
+        for (int i = t; i < n * W; i += gen_nth<NTMAX>())
         {
             const int e = gdiv(i, W, inv_w), w = i - e * W;
             if (w < nvalid) atomicMax(&smax[w], __float_as_uint(absmax(X[w * ST + e])));
         }
         __syncthreads();
     }
-    for (int i = t; i < n * W; i += GEN_NT)
+    for (int i = t; i < n * W; i += gen_nth<NTMAX>())
     {
         const int e = gdiv(i, W, inv_w), w = i - e * W;
         if (w >= nvalid) continue;
@@ -438,8 +447,8 @@ __global__ void __launch_bounds__(GEN_NT) gen_col_kernel(ColArgs a, GenPlan g, c
 //   H = G on 0<k<L (halved when HALVE), H[0] = Re G[0], H[L] = Re G[L],
 //   Z[k] = (H[k] + conj H[L-k]) + i (H[k] - conj H[L-k]) conj(w_{Ny}^k),
 //   out[2n] + i out[2n+1] = IDFT_L(Z) = conj(DFT(conj Z)).
-template <typename T, bool HALVE>
-__global__ void __launch_bounds__(GEN_NT) gen_row_c2r_kernel(RowArgs a, GenPlan g, const cpx<double> *tabL,
+template <typename T, bool HALVE, int NTMAX = GEN_NT>
+__global__ void __launch_bounds__(NTMAX) gen_row_c2r_kernel(RowArgs a, GenPlan g, const cpx<double> *tabL,
                                                             const cpx<double> *tab2, int nseq)
 {
     cpx<T> *buf = gen_lds<T>();
@@ -451,7 +460,7 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_c2r_kernel(RowArgs a, GenPlan 
     T *out = reinterpret_cast<T *>(a.out) + b * a.dist_out + (long long)row0 * a.ld_out;
     const T hs = HALVE ? T(0.5) : T(1);
     const float inv_l = 1.0f / (float)L;
-    for (int i = t; i < nseq * L; i += GEN_NT)
+    for (int i = t; i < nseq * L; i += gen_nth<NTMAX>())
     {
         const int s = gdiv(i, L, inv_l), k = i - s * L;
         cpx<T> z = {T(0), T(0)};
@@ -475,8 +484,8 @@ __global__ void __launch_bounds__(GEN_NT) gen_row_c2r_kernel(RowArgs a, GenPlan 
         buf[s * ST + k] = conjc(z);
     }
     __syncthreads();
-    const cpx<T> *U = gen_fft<T>(buf, buf + nseq * ST, nseq, g, tabL);
-    for (int i = t; i < nrow * L; i += GEN_NT)
+    const cpx<T> *U = gen_fft<T, NTMAX>(buf, buf + nseq * ST, nseq, g, tabL);
+    for (int i = t; i < nrow * L; i += gen_nth<NTMAX>())
     {
         const int s = gdiv(i, L, inv_l), e = i - s * L;
         const cpx<T> u = U[s * ST + e];
@@ -525,8 +534,8 @@ __device__ __forceinline__ void gen_ew(int i, int n, int W, bool seq_fast, float
     }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(GEN_NT) gen_strided_kernel(GenStride a, GenPlan g, const cpx<double> *tab,
+template <typename T, int NTMAX = GEN_NT>
+__global__ void __launch_bounds__(NTMAX) gen_strided_kernel(GenStride a, GenPlan g, const cpx<double> *tab,
                                                             const cpx<double> *twt, const cpx<double> *tab2, int W)
 {
     cpx<T> *buf = gen_lds<T>();
@@ -538,7 +547,7 @@ __global__ void __launch_bounds__(GEN_NT) gen_strided_kernel(GenStride a, GenPla
     T *dst = reinterpret_cast<T *>(a.out) + 2 * (bat * a.out_bat + sub * a.out_sub + (long long)col0 * a.out_col);
     const float inv_n = 1.0f / (float)n, inv_w = 1.0f / (float)W;
     const bool in_fast = a.in_seq == 1, out_fast = a.out_seq == 1;
-    for (int i = t; i < n * W; i += GEN_NT)
+    for (int i = t; i < n * W; i += gen_nth<NTMAX>())
     {
         int e, w;
         gen_ew(i, n, W, in_fast, inv_n, inv_w, e, w);
@@ -571,8 +580,8 @@ __global__ void __launch_bounds__(GEN_NT) gen_strided_kernel(GenStride a, GenPla
         buf[w * ST + e] = z;
     }
     __syncthreads();
-    const cpx<T> *X = gen_fft<T>(buf, buf + ST * W, W, g, tab);
-    for (int i = t; i < n * W; i += GEN_NT)
+    const cpx<T> *X = gen_fft<T, NTMAX>(buf, buf + ST * W, W, g, tab);
+    for (int i = t; i < n * W; i += gen_nth<NTMAX>())
     {
         int k, w;
         gen_ew(i, n, W, out_fast, inv_n, inv_w, k, w);
