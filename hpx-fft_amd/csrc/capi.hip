@@ -48,6 +48,27 @@ int fail(int code, const std::string &msg)
     return code;
 }
 
+// run an entry point's body; no exception crosses the C ABI
+template <typename F> int guarded(F &&body)
+{
+    try
+    {
+        return body();
+    }
+    catch (const std::bad_alloc &)
+    {
+        return fail(HPXFFT_OUT_OF_MEMORY, "host allocation failed");
+    }
+    catch (const std::exception &e)
+    {
+        return fail(HPXFFT_HIP_ERROR, e.what());
+    }
+    catch (...)
+    {
+        return fail(HPXFFT_HIP_ERROR, "unexpected exception");
+    }
+}
+
 #define HIPCHK(x)                                                                                                  \
     do                                                                                                             \
     {                                                                                                              \
@@ -1404,85 +1425,97 @@ extern "C" {
 
 int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_real)
 {
-    return exec_backward(plan, d_spec, d_real, true);
+    return guarded([&]() -> int {
+        return exec_backward(plan, d_spec, d_real, true);
+    });
 }
 
 int hpxfft_exec_c2r_2d(hpxfft_plan plan, const void *d_spec, void *d_real)
 {
-    return exec_backward(plan, d_spec, d_real, false);
+    return guarded([&]() -> int {
+        return exec_backward(plan, d_spec, d_real, false);
+    });
 }
 
 int hpxfft_get_measurement(hpxfft_plan plan, const char *key, double *value)
 {
-    if (!plan || !key || !value) return fail(HPXFFT_INVALID_ARG, "null argument");
-    const std::string k(key);
-    double v = 0.0;
-    if (k == "plan") v = plan->plan_seconds;
-    else if (k == "plan_flops") v = plan->plan_flops;
-    else if (k == "first_trans" || k == "second_trans") v = 0.0;
-    else if (k == "h2d") v = plan->h2d;
-    else if (k == "d2h") v = plan->d2h;
-    else if (k == "total" || k == "first_fftw" || k == "second_fftw")
-    {
-        if (plan->ran)
+    return guarded([&]() -> int {
+        if (!plan || !key || !value) return fail(HPXFFT_INVALID_ARG, "null argument");
+        const std::string k(key);
+        double v = 0.0;
+        if (k == "plan") v = plan->plan_seconds;
+        else if (k == "plan_flops") v = plan->plan_flops;
+        else if (k == "first_trans" || k == "second_trans") v = 0.0;
+        else if (k == "h2d") v = plan->h2d;
+        else if (k == "d2h") v = plan->d2h;
+        else if (k == "total" || k == "first_fftw" || k == "second_fftw")
         {
-            device_guard g(plan->device);
-            HIPCHK(hipEventSynchronize(plan->ev[2]));
-            if (k == "total") v = elapsed(plan->ev[0], plan->ev[2]);
-            else if (k == "first_fftw") v = elapsed(plan->ev[0], plan->ev[1]);
-            else v = elapsed(plan->ev[1], plan->ev[2]);
+            if (plan->ran)
+            {
+                device_guard g(plan->device);
+                HIPCHK(hipEventSynchronize(plan->ev[2]));
+                if (k == "total") v = elapsed(plan->ev[0], plan->ev[2]);
+                else if (k == "first_fftw") v = elapsed(plan->ev[0], plan->ev[1]);
+                else v = elapsed(plan->ev[1], plan->ev[2]);
+            }
         }
-    }
-    else if (k == "column_a" || k == "column_b")
-    {
-        // the two launches of a four-step column pass (0 when the last run had no split)
-        if (plan->ran && plan->col_split)
+        else if (k == "column_a" || k == "column_b")
         {
-            device_guard g(plan->device);
-            HIPCHK(hipEventSynchronize(plan->ev[2]));
-            v = k == "column_a" ? elapsed(plan->ev[1], plan->ev[3]) : elapsed(plan->ev[3], plan->ev[2]);
+            // the two launches of a four-step column pass (0 when the last run had no split)
+            if (plan->ran && plan->col_split)
+            {
+                device_guard g(plan->device);
+                HIPCHK(hipEventSynchronize(plan->ev[2]));
+                v = k == "column_a" ? elapsed(plan->ev[1], plan->ev[3]) : elapsed(plan->ev[3], plan->ev[2]);
+            }
         }
-    }
-    else
-    {
-        auto it = plan->extra.find(k);
-        if (it != plan->extra.end()) v = it->second;
-        else plan->extra[k] = 0.0;
-    }
-    *value = v;
-    return HPXFFT_OK;
+        else
+        {
+            auto it = plan->extra.find(k);
+            if (it != plan->extra.end()) v = it->second;
+            else plan->extra[k] = 0.0;
+        }
+        *value = v;
+        return HPXFFT_OK;
+    });
 }
 
 int hpxfft_plan_describe(hpxfft_plan plan, char *buf, size_t len, size_t *needed)
 {
-    if (!plan) return fail(HPXFFT_INVALID_ARG, "null plan");
-    const std::string s = describe(plan);
-    if (needed) *needed = s.size();
-    if (buf && len)
-    {
-        const size_t n = std::min(len - 1, s.size());
-        std::memcpy(buf, s.data(), n);
-        buf[n] = '\0';
-    }
-    return HPXFFT_OK;
+    return guarded([&]() -> int {
+        if (!plan) return fail(HPXFFT_INVALID_ARG, "null plan");
+        const std::string s = describe(plan);
+        if (needed) *needed = s.size();
+        if (buf && len)
+        {
+            const size_t n = std::min(len - 1, s.size());
+            std::memcpy(buf, s.data(), n);
+            buf[n] = '\0';
+        }
+        return HPXFFT_OK;
+    });
 }
 
 int hpxfft_write_plan(hpxfft_plan plan, const char *path)
 {
-    if (!plan || !path) return fail(HPXFFT_INVALID_ARG, "null argument");
-    FILE *f = std::fopen(path, "a");
-    if (!f) return fail(HPXFFT_IO_ERROR, std::string("Failed to open file: ") + path);
-    const std::string s = describe(plan);
-    std::fprintf(f, "GPU r2c 2D plan:\n%s\n", s.c_str());
-    std::fclose(f);
-    return HPXFFT_OK;
+    return guarded([&]() -> int {
+        if (!plan || !path) return fail(HPXFFT_INVALID_ARG, "null argument");
+        FILE *f = std::fopen(path, "a");
+        if (!f) return fail(HPXFFT_IO_ERROR, std::string("Failed to open file: ") + path);
+        const std::string s = describe(plan);
+        std::fprintf(f, "GPU r2c 2D plan:\n%s\n", s.c_str());
+        std::fclose(f);
+        return HPXFFT_OK;
+    });
 }
 
 int hpxfft_plan_destroy(hpxfft_plan plan)
 {
-    if (!plan) return HPXFFT_OK;
-    free_plan(plan);
-    return HPXFFT_OK;
+    return guarded([&]() -> int {
+        if (!plan) return HPXFFT_OK;
+        free_plan(plan);
+        return HPXFFT_OK;
+    });
 }
 
 }   // extern "C"
@@ -1697,18 +1730,20 @@ int hpxfft_plan_batched_set_config(hpxfft_plan plan, int row_cfg, int col_cfg)
 
 int hpxfft_exec_r2c_2d_batched(hpxfft_plan plan, void *const *d_grids)
 {
-    if (!plan || !plan->batched) return fail(HPXFFT_INVALID_ARG, "not a batched plan");
-    if (!d_grids) return fail(HPXFFT_INVALID_ARG, "null grid pointer array");
-    device_guard g(plan->device);
-    // the pinned descriptor staging buffer may still be read by a previous upload
-    HIPCHK(hipStreamSynchronize(plan->stream));
-    if (int rc = batched_upload_grids(plan, d_grids)) return rc;
-    HIPCHK(hipEventRecord(plan->ev[0], plan->stream));
-    if (int rc = batched_run(plan, plan->bcfg[0], plan->bcfg[1])) return rc;
-    HIPCHK(hipEventRecord(plan->ev[2], plan->stream));
-    plan->ran = true;
-    plan->col_split = false;
-    return HPXFFT_OK;
+    return guarded([&]() -> int {
+        if (!plan || !plan->batched) return fail(HPXFFT_INVALID_ARG, "not a batched plan");
+        if (!d_grids) return fail(HPXFFT_INVALID_ARG, "null grid pointer array");
+        device_guard g(plan->device);
+        // the pinned descriptor staging buffer may still be read by a previous upload
+        HIPCHK(hipStreamSynchronize(plan->stream));
+        if (int rc = batched_upload_grids(plan, d_grids)) return rc;
+        HIPCHK(hipEventRecord(plan->ev[0], plan->stream));
+        if (int rc = batched_run(plan, plan->bcfg[0], plan->bcfg[1])) return rc;
+        HIPCHK(hipEventRecord(plan->ev[2], plan->stream));
+        plan->ran = true;
+        plan->col_split = false;
+        return HPXFFT_OK;
+    });
 }
 
 }   // extern "C"
