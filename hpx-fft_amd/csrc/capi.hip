@@ -1019,6 +1019,19 @@ int plan_common(hpxfft_plan_s *p)
     const std::string dims = std::to_string(p->nx) + " x " + std::to_string(p->ny);
     if (p->nx == 0 || p->ny < 2 || p->ny % 2)
         return fail(HPXFFT_UNSUPPORTED_SIZE, "need Nx >= 1 and an even Ny >= 2 (got " + dims + ")");
+    // grids run along the second or third launch dimension
+    if (p->batch > 65535)
+        return fail(HPXFFT_UNSUPPORTED_SIZE, "at most 65535 grids per plan (got " + std::to_string(p->batch) + ")");
+    {
+        // every element offset the kernels form fits a signed 64-bit index
+        const unsigned __int128 lim = (unsigned __int128)1 << 62;
+        const unsigned __int128 ext_in = (unsigned __int128)(p->batch - 1) * p->dist_in +
+                                         (unsigned __int128)p->nx * p->ld_in;
+        const unsigned __int128 ext_out =
+            2 * ((unsigned __int128)(p->batch - 1) * p->dist_out + (unsigned __int128)p->nx * p->ld_out);
+        if (p->ld_in >= lim || p->ld_out >= lim || ext_in >= lim || ext_out >= lim)
+            return fail(HPXFFT_INVALID_ARG, "layout: the strides address more than 2^62 elements");
+    }
     // the general kernels take lengths that are not powers of two, and
     // powers of two beyond the power-of-two kernels (Nx > 2^16, Ny > 2^15)
     p->gen_row = !is_pow2(p->ny) || p->ny / 2 > (size_t(1) << ROW_MAX_LG);
@@ -1087,7 +1100,6 @@ int plan_common(hpxfft_plan_s *p)
     if (p->lgL > ROW_MAX_LG || p->lgx > 2 * COL_TILE_MAX_LG)
         return fail(HPXFFT_UNSUPPORTED_SIZE, "grid too large: Nx <= 65536 and Ny <= 32768 are supported");
     p->ncols = p->ny / 2 + 1;
-    const size_t sb = storage_bytes(p->prec);
     if (p->ld_in < p->ny || p->ld_out < p->ncols || (p->batch > 1 && (p->dist_in < p->nx * p->ld_in ||
                                                                       p->dist_out < p->nx * p->ld_out)))
         return fail(HPXFFT_INVALID_ARG, "layout: need ld_in >= Ny, ld_out >= Ny/2+1 and non-overlapping grids");
@@ -1173,7 +1185,6 @@ int plan_common(hpxfft_plan_s *p)
             HIPCHK(hipMalloc(&p->colmax, sizeof(unsigned) * p->ncols * p->batch));
         }
     }
-    (void)sb;
     if (int rc = alloc_events(p)) return rc;
     if (p->four_step && p->effort >= HPXFFT_MEASURE)
         if (int rc = tune_split(p)) return rc;

@@ -131,6 +131,13 @@ def test_bad_arguments():
     assert plan_status(16, 16, ld_out=4) == _lib.INVALID_ARG         # ld_out < Ny/2+1
     assert plan_status(16, 16, ld_in=17) == _lib.INVALID_ARG         # odd pitch
     assert plan_status(16, 16, batch=2, dist_in=8) == _lib.INVALID_ARG
+    # grids beyond a launch dimension, and strides past a 64-bit element index
+    assert plan_status(16, 16, batch=65536) == _lib.UNSUPPORTED_SIZE
+    assert "65535 grids" in hpxfft_amd.last_error()
+    assert plan_status(16, 16, ld_in=1 << 61, dist_in=0) == _lib.INVALID_ARG
+    assert "2^62" in hpxfft_amd.last_error()
+    assert plan_status(16, 16, batch=2, dist_in=1 << 62) == _lib.INVALID_ARG
+    assert plan_status(16, 16, batch=3, ld_out=1 << 60, dist_out=1 << 61, ld_in=18, dist_in=288) == _lib.INVALID_ARG
     assert L.hpxfft_plan_r2c_2d_many(None, 16, 16, 1, 18, 288, 9, 144, 0, 0, None) == _lib.INVALID_ARG
     assert L.hpxfft_exec_r2c_2d(None, None, None) == _lib.INVALID_ARG
     assert L.hpxfft_exec_r2c_2d_adjoint(None, None, None) == _lib.INVALID_ARG
