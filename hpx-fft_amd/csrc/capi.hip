@@ -13,6 +13,7 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -859,9 +860,21 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
     return HPXFFT_OK;
 }
 
+// the kernels move whole complex elements (real rows as complex pairs), so
+// both device pointers must be aligned to one complex element of the storage
+int check_aligned(const hpxfft_plan_s *p, const void *a, const void *b)
+{
+    const size_t al = 2 * storage_bytes(p->prec);
+    if ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) % al)
+        return fail(HPXFFT_INVALID_ARG, "device pointers must be aligned to one complex element (" +
+                                            std::to_string(al) + " bytes) of the plan's storage");
+    return HPXFFT_OK;
+}
+
 int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_exp, int *col_exp)
 {
     if (!d_in || !d_out) return fail(HPXFFT_INVALID_ARG, "null device pointer");
+    if (int rc = check_aligned(p, d_in, d_out)) return rc;
     if (d_in == d_out && (p->ld_in != 2 * p->ld_out || p->dist_in != 2 * p->dist_out))
         return fail(HPXFFT_INVALID_ARG, "in-place exec needs ld_in == 2*ld_out and dist_in == 2*dist_out");
     const bool s16 = p->prec == P_F16 || p->prec == P_BF16;
@@ -1368,6 +1381,7 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     const char *what = halve ? "adjoint" : "c2r";
     if (!plan || plan->batched) return fail(HPXFFT_INVALID_ARG, "invalid plan");
     if (!d_spec || !d_real) return fail(HPXFFT_INVALID_ARG, "null device pointer");
+    if (int rc = check_aligned(plan, d_spec, d_real)) return rc;
     if (plan->prec != P_F64 && plan->prec != P_F32)
         return fail(HPXFFT_INVALID_ARG, std::string(what) + " is implemented for F64 and F32 plans");
     device_guard g(plan->device);
@@ -1556,6 +1570,7 @@ int batched_upload_grids(hpxfft_plan_s *p, void *const *d_grids)
     for (size_t i = 0; i < p->bnx.size(); ++i)
     {
         if (!d_grids[i]) return fail(HPXFFT_INVALID_ARG, "null grid pointer " + std::to_string(i));
+        if (int rc = check_aligned(p, d_grids[i], d_grids[i])) return rc;
         batched_fill_grid(st + i * gb, reinterpret_cast<float *>(d_grids[i]), (int)p->bnx[i], (int)p->bny[i]);
     }
     HIPCHK(hipMemcpyAsync(p->bgrids, p->bstage, gb * p->bnx.size(), hipMemcpyHostToDevice, p->stream));

@@ -65,10 +65,18 @@ def _check_real(x: torch.Tensor):
         raise ValueError("need (..., Nx, Ny)")
 
 
+def _on_complex_element(t: torch.Tensor) -> torch.Tensor:
+    """Contiguous, starting on a whole complex element (the kernels' unit):
+    a view that starts one real in (storage offset) is copied."""
+    t = t.contiguous()
+    unit = t.element_size() * (1 if t.is_complex() else 2)
+    return t if t.data_ptr() % unit == 0 else t.clone()
+
+
 def rfft2(x: torch.Tensor) -> torch.Tensor:
     """X[..., kx, ky] = sum_{m,n} x[..., m, n] exp(-2 pi i (kx m/Nx + ky n/Ny)), ky <= Ny/2."""
     _check_real(x)
-    x = x.contiguous()
+    x = _on_complex_element(x)
     nx, ny = x.shape[-2], x.shape[-1]
     batch = x.numel() // (nx * ny) if x.numel() else 0
     prec, cdt = _DT[x.dtype]
@@ -98,7 +106,7 @@ def _backward(g: torch.Tensor, ny: int, adjoint: bool) -> torch.Tensor:
     rdt = {torch.complex128: torch.float64, torch.complex64: torch.float32}.get(g.dtype)
     if rdt is None:
         raise TypeError(f"unsupported dtype {g.dtype}")
-    g = g.contiguous()
+    g = _on_complex_element(g)
     nx = g.shape[-2]
     if g.shape[-1] != ny // 2 + 1:
         raise ValueError("last dim must be Ny/2+1")

@@ -93,7 +93,9 @@ int hpxfft_plan_r2c_2d_many(hpxfft_plan *plan, size_t nx, size_t ny, size_t batc
  * (core/src/shared/loop.cpp:56-155): row r2c, transpose, column c2c,
  * transpose back, on device memory, enqueued on the plan's stream (no host
  * synchronisation).  d_in/d_out are device pointers of the plan's storage
- * type (double, float, _Float16 or bfloat16 bit patterns). */
+ * type (double, float, _Float16 or bfloat16 bit patterns), each aligned to
+ * one complex element (16, 8 or 4 bytes; HPXFFT_INVALID_ARG otherwise, as
+ * for every exec function below). */
 int hpxfft_exec_r2c_2d(hpxfft_plan plan, const void *d_in, void *d_out);
 
 /* 16-bit storage plans: value = stored * 2^-exp.  d_in_row_exp (optional,
@@ -118,10 +120,10 @@ int hpxfft_exec_r2c_2d_host(hpxfft_plan plan, double *inout);
  * The first backward call on a plan (this or hpxfft_exec_c2r_2d) allocates
  * its work buffer, one spectrum; later calls allocate nothing.
  *
- * Every exec function enqueues only kernels and event records on the plan's
- * stream, so a caller may capture it into a HIP graph once that first
- * allocation is done (make one backward call before capturing a backward
- * one). */
+ * Every exec function except the host path and the batched one enqueues
+ * only kernels and event records on the plan's stream, so a caller may
+ * capture it into a HIP graph once that first allocation is done (make one
+ * backward call before capturing a backward one). */
 int hpxfft_exec_r2c_2d_adjoint(hpxfft_plan plan, const void *d_spec, void *d_real);
 
 /* Inverse 2-D c2r, unnormalised, in FFTW's FFTW_BACKWARD convention: the
@@ -164,7 +166,9 @@ int hpxfft_plan_destroy(hpxfft_plan plan);
 int hpxfft_plan_r2c_2d_batched(hpxfft_plan *plan, size_t count, const size_t *nx, const size_t *ny, int precision,
                                int effort, void *stream);
 
-/* d_grids: host array of `count` device pointers, one per grid (in place). */
+/* d_grids: host array of `count` device pointers, one per grid (in place).
+ * Waits for the plan's previous batched exec, then uploads the pointers
+ * (a host-to-device copy on the plan's stream): not for graph capture. */
 int hpxfft_exec_r2c_2d_batched(hpxfft_plan plan, void *const *d_grids);
 
 /* No reference counterpart: overrides the task shape a batched plan runs in

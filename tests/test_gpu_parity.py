@@ -63,6 +63,41 @@ def test_parity_f64(gpu, nx, ny):
     assert err <= TOL64, err
 
 
+@pytest.mark.parametrize("prec,dt", [("f64", "float64"), ("f32", "float32")])
+def test_pointers_off_a_complex_element(gpu, prec, dt):
+    """Device pointers must start on a whole complex element: the C ABI
+    rejects one real off with INVALID_ARG (checked on the host, nothing is
+    launched), and rfft2 copies such a view before the transform."""
+    import torch
+    import hpxfft_amd
+    from hpxfft_amd import _lib
+    nx, ny = 16, 32
+    ncy = ny // 2 + 1
+    tdt = getattr(torch, dt)
+    base = torch.zeros(nx * ny + 1, dtype=tdt, device="cuda")
+    out = torch.zeros(nx * ncy * 2 + 2, dtype=tdt, device="cuda")
+    p = hpxfft_amd.Plan(nx, ny, ld_in=ny, ld_out=ncy, precision=prec)
+    es = base.element_size()
+    for d_in, d_out in [(base.data_ptr() + es, out.data_ptr()), (base.data_ptr(), out.data_ptr() + es),
+                        (base.data_ptr() + es, base.data_ptr() + es)]:
+        with pytest.raises(RuntimeError) as e:
+            p.exec(d_in, d_out)
+        assert e.value.status == _lib.INVALID_ARG
+        assert "aligned" in hpxfft_amd.last_error()
+        with pytest.raises(RuntimeError) as e:
+            p.exec_c2r(d_out, d_in)
+        assert e.value.status == _lib.INVALID_ARG
+    p.close()
+    assert torch.count_nonzero(out) == 0
+    x = oracle.fill(nx, ny, seed=5, dist="upm1")[:, :ny].copy()
+    base[1:] = torch.from_numpy(x.reshape(-1)).to(device="cuda", dtype=tdt)
+    view = base[1:].view(nx, ny)
+    assert view.data_ptr() % (2 * es) != 0
+    got = hpxfft_amd.rfft2(view).cpu().numpy().astype(np.complex128)
+    err = oracle.rel_l2(got, oracle.r2c_2d(x))
+    assert err <= (TOL64 if prec == "f64" else TOL32), err
+
+
 @pytest.mark.parametrize("nx,ny,batch", [(128, 256, 3), (1024, 64, 2), (4096, 128, 3), (16, 4096, 5),
                                          # general one-level, Bluestein and two-level lengths
                                          (30, 42, 3), (67, 134, 2), (6000, 20, 2), (12, 2 * 6000, 2)])
