@@ -245,6 +245,7 @@ struct hpxfft_plan_s
     void *gtmp = nullptr;          // two-level rows: phase A output, Nx x Ny/2 per grid (compute type)
     void *gscr = nullptr;          // two-level columns: phase A output, Nx x (Ny/2+1) per grid
     double gcol_ms[2] = {};        // measure effort: general column pass ms, one level / two levels
+    double gip_ms[2][2] = {};      // measure effort: general [row, column] pass ms, [ping-pong, in place]
     double *gen_tab = nullptr;     // exact FP64 tables w_n^m: rows L = Ny/2, split Ny, columns Nx
     size_t gtab_L = 0, gtab_2 = 0, gtab_N = 0;   // their offsets (complex entries)
     // device memory
@@ -292,9 +293,9 @@ const void *tw64_ptr(const hpxfft_plan_s *p, size_t N) { return p->tabs.tw64 + 2
 
 // dynamic LDS of a general kernel: two buffers of `points` complex values in
 // the compute type, then `words` 32-bit words (16-bit storage: row / column maxima)
-size_t gen_lds_bytes(const hpxfft_plan_s *p, size_t points, size_t words = 0)
+size_t gen_lds_bytes(const hpxfft_plan_s *p, const hpxfft_dev::GenPlan &g, size_t points, size_t words = 0)
 {
-    return 2 * points * 2 * compute_bytes(p->prec) + 4 * words;
+    return (size_t)hpxfft_dev::gen_nbuf(g) * points * 2 * compute_bytes(p->prec) + 4 * words;
 }
 
 // threads of a general workgroup whose dynamic LDS is `lds` bytes: 256 while
@@ -383,6 +384,16 @@ bool gen_setup(size_t n, size_t cap, hpxfft_dev::GenPlan &g)
     {
         g.len = (int)n;
         g.off_c = g.off_b = 0;
+        // register radices only: in-place stages, their input positions
+        // after the n twiddles.  In place is the estimate wherever the
+        // radices allow it (one buffer instead of two; measure effort times
+        // both forms, tune_gen_ip): 3000 x 3000 FP64 columns of 96 KiB per
+        // workgroup 117 -> 79 us, rows of 1500 points 70 -> 58 us; rows of
+        // 4 x 500 points 16 -> 17 us, the one case measured slower
+        bool small = true;
+        for (int i = 0; i < g.nstages; ++i) small = small && g.radix[i] <= 7;
+        g.off_r = small ? g.n : -1;
+        g.ip = small ? 1 : 0;
         return true;
     }
     size_t M = 1;
@@ -391,8 +402,11 @@ bool gen_setup(size_t n, size_t cap, hpxfft_dev::GenPlan &g)
     g.len = (int)n;
     g.off_c = (int)M;
     g.off_b = (int)(M + n);
+    g.off_r = -1;
+    g.ip = 0;
     return true;
 }
+
 
 // two-level split N = P * Q (N <= 2^20): both sub-lengths within one
 // workgroup, direct plans preferred over Bluestein, then the most balanced
@@ -418,6 +432,10 @@ bool gen2_setup(size_t N, size_t cap, gen2_dim &d)
         }
     }
     if (best < 0) return false;
+    // the phases hold several short sequences per workgroup, well within the
+    // LDS: ping-pong stages (in place measured slower: 10000 x 10000 FP32
+    // columns, 100 * 100, 0.69 -> 0.74 ms)
+    d.A.ip = d.B.ip = 0;
     d.on = d.avail = true;
     d.P = best;
     d.Q = (int)(N / (size_t)best);
@@ -432,9 +450,29 @@ int gen_width(const hpxfft_dev::GenPlan &g, size_t cap, size_t wmax)
     return (int)w;
 }
 
-// table entries of a general plan: w_n^m (m < n), and for Bluestein the chirp
-// (len) and the kernel spectrum (n)
-size_t gen_table_size(const hpxfft_dev::GenPlan &g) { return g.len == g.n ? (size_t)g.n : (size_t)(2 * g.n + g.len); }
+// table entries (complex FP64) of a general plan: w_n^m (m < n); for
+// Bluestein the chirp (len) and the kernel spectrum (n); for in-place plans
+// the n int input positions, four per entry
+size_t gen_table_size(const hpxfft_dev::GenPlan &g)
+{
+    if (g.len != g.n) return (size_t)(2 * g.n + g.len);
+    return (size_t)g.n + (g.off_r >= 0 ? ((size_t)g.n + 3) / 4 : 0);
+}
+
+// input position of element e for in-place stages r_1 .. r_S: the digits of
+// e = d_S + r_S (d_{S-1} + ... + r_2 d_1), weighted d_s -> r_1 ... r_{s-1}
+int gen_digit_rev(const hpxfft_dev::GenPlan &g, int e)
+{
+    int pos = 0, w = g.n;
+    for (int s = g.nstages - 1; s >= 0; --s)
+    {
+        const int r = g.radix[s];
+        w /= r;
+        pos += (e % r) * w;
+        e /= r;
+    }
+    return pos;
+}
 
 // fill a general plan's table at h[2*off ...] (complex FP64 pairs), from long
 // double values: w_n^m exact (octant-reduced), the chirp c[j] = w_{2len}^{j^2
@@ -449,6 +487,12 @@ void gen_table(const hpxfft_dev::GenPlan &g, std::vector<double> &h, size_t off)
         twiddle_ld(m, n, c, sn);
         h[2 * (off + m)] = (double)c;
         h[2 * (off + m) + 1] = -(double)sn;
+    }
+    if (g.off_r >= 0)
+    {
+        std::vector<int> pos(((size_t)n + 3) / 4 * 4, 0);
+        for (uint64_t e = 0; e < n; ++e) pos[e] = gen_digit_rev(g, (int)e);
+        std::memcpy(&h[2 * (off + (size_t)g.off_r)], pos.data(), sizeof(int) * pos.size());
     }
     if (g.len == g.n) return;
     const uint64_t len = (uint64_t)g.len;
@@ -519,6 +563,7 @@ std::string radices(const hpxfft_dev::GenPlan &g)
 
 std::string gen_method(const hpxfft_dev::GenPlan &g)
 {
+    if (g.ip) return "mixed-radix in-place decimation in time in LDS, radices " + radices(g);
     if (g.len == g.n) return "mixed-radix Stockham in LDS, radices " + radices(g);
     return "Bluestein (chirp convolution through a " + std::to_string(g.n) + "-point FFT in LDS, radices " +
            radices(g) + ")";
@@ -564,7 +609,7 @@ std::string describe(const hpxfft_plan_s *p)
     else if (p->gen_row)
         os << "  row r2c kernel (general length): " << p->ny / 2 << "-point " << gen_method(p->grow) << ", "
            << p->gen_rows << " row(s)/workgroup, "
-           << gen_threads(gen_lds_bytes(p, (size_t)p->gen_rows * p->grow.n, (size_t)p->gen_rows), p->grow)
+           << gen_threads(gen_lds_bytes(p, p->grow, (size_t)p->gen_rows * p->grow.n, (size_t)p->gen_rows), p->grow)
            << " threads + r2c split\n";
     else
     {
@@ -573,6 +618,10 @@ std::string describe(const hpxfft_plan_s *p)
            << rg.nseq << " row(s)/workgroup, " << rg.nt << " threads, " << rg.E << " elements/thread, radix "
            << (rg.E < 16 ? rg.E : 16) << (rg.split ? ", LDS exchange in re/im halves" : "") << "\n";
     }
+    for (int d = 0; d < 2; ++d)
+        if (p->gip_ms[d][0] > 0.0)
+            os << "  general " << (d ? "column" : "row") << " pass autotuned (ms): ping-pong " << p->gip_ms[d][0]
+               << ", in place " << p->gip_ms[d][1] << "\n";
     if (p->gcol_ms[0] > 0.0)
         os << "  general column pass autotuned (ms): one level " << p->gcol_ms[0] << ", two levels " << p->gcol_ms[1]
            << "\n";
@@ -633,7 +682,7 @@ int launch_strided(hpxfft_plan_s *p, const hpxfft_dev::GenStride &a, const hpxff
                    size_t twt_off, int W, unsigned gx, unsigned gy, unsigned gz)
 {
     const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
-    const size_t lds = gen_lds_bytes(p, (size_t)W * g.n);
+    const size_t lds = gen_lds_bytes(p, g, (size_t)W * g.n);
     const dim3 grid(gx, gy, gz);
     const unsigned nt = gen_threads(lds, g);
     if (p->prec == P_F64)
@@ -784,7 +833,7 @@ int run_columns(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj
         a.in = spec_in;
         a.out = spec_out;
         const dim3 grid((unsigned)((p->ncols + p->gen_w - 1) / p->gen_w), (unsigned)p->batch);
-        const size_t lds = gen_lds_bytes(p, (size_t)p->gen_w * p->gcol.n, (size_t)p->gen_w);
+        const size_t lds = gen_lds_bytes(p, p->gcol, (size_t)p->gen_w * p->gcol.n, (size_t)p->gen_w);
         const unsigned nt = gen_threads(lds, p->gcol);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab) + p->gtab_N;
         using namespace hpxfft_dev;
@@ -902,7 +951,7 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     else if (p->gen_row)
     {
         const int ns = p->gen_rows;
-        const size_t lds = gen_lds_bytes(p, (size_t)ns * p->grow.n, (size_t)ns);
+        const size_t lds = gen_lds_bytes(p, p->grow, (size_t)ns * p->grow.n, (size_t)ns);
         const dim3 grid((unsigned)((p->nx + ns - 1) / ns), (unsigned)p->batch);
         const unsigned nt = gen_threads(lds, p->grow);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
@@ -984,6 +1033,59 @@ int tune_split(hpxfft_plan_s *p)
     p->lgQ = p->lgx - best;
     if (rc != HPXFFT_OK) return rc;
     HIPCHK(e);
+    return HPXFFT_OK;
+}
+
+// measure effort, one-level general rows / columns whose radices allow it:
+// time the forward transform with the ping-pong and the in-place stages on
+// zeroed buffers of the plan's layout (the row and column passes are timed
+// apart by the exec's own events) and keep the faster form of each pass.
+// If the buffers cannot be allocated the estimate stays.
+int tune_gen_ip(hpxfft_plan_s *p)
+{
+    const size_t sb = storage_bytes(p->prec);
+    const size_t bin = p->batch * p->dist_in * sb, bout = p->batch * p->dist_out * 2 * sb;
+    void *in = nullptr, *out = nullptr;
+    if (hipMalloc(&in, bin) != hipSuccess || hipMalloc(&out, bout) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        (void)hipFree(in);
+        return HPXFFT_OK;
+    }
+    const int ip_row = p->grow.ip, ip_col = p->gcol.ip;
+    const bool row_ok = p->gen_row && !p->g2r.on && p->grow.off_r >= 0;
+    const bool col_ok = p->gen_col && !p->g2c.on && p->gcol.off_r >= 0;
+    int rc = HPXFFT_OK;
+    hipError_t e = hipMemsetAsync(in, 0, bin, p->stream);
+    for (int ip = 0; ip < 2 && e == hipSuccess && rc == HPXFFT_OK; ++ip)
+    {
+        if (row_ok) p->grow.ip = ip;
+        if (col_ok) p->gcol.ip = ip;
+        constexpr int REPS = 3;
+        double row_s = 0.0, col_s = 0.0;
+        for (int r = 0; r <= REPS && rc == HPXFFT_OK && e == hipSuccess; ++r)   // r = 0: warm-up
+        {
+            rc = exec_forward(p, in, out, nullptr, nullptr);
+            if (rc == HPXFFT_OK) e = hipEventSynchronize(p->ev[2]);
+            if (rc == HPXFFT_OK && e == hipSuccess && r > 0)
+            {
+                row_s += elapsed(p->ev[0], p->ev[1]);
+                col_s += elapsed(p->ev[1], p->ev[2]);
+            }
+        }
+        p->gip_ms[0][ip] = row_ok ? row_s * 1e3 / REPS : 0.0;
+        p->gip_ms[1][ip] = col_ok ? col_s * 1e3 / REPS : 0.0;
+    }
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipFree(in);
+    (void)hipFree(out);
+    p->ran = false;
+    p->grow.ip = ip_row;
+    p->gcol.ip = ip_col;
+    if (rc != HPXFFT_OK) return rc;
+    HIPCHK(e);
+    if (row_ok) p->grow.ip = p->gip_ms[0][1] < p->gip_ms[0][0] ? 1 : 0;
+    if (col_ok) p->gcol.ip = p->gip_ms[1][1] < p->gip_ms[1][0] ? 1 : 0;
     return HPXFFT_OK;
 }
 
@@ -1201,6 +1303,9 @@ int plan_common(hpxfft_plan_s *p)
     if (int rc = alloc_events(p)) return rc;
     if (p->four_step && p->effort >= HPXFFT_MEASURE)
         if (int rc = tune_split(p)) return rc;
+    if (p->effort >= HPXFFT_MEASURE && ((p->gen_row && !p->g2r.on && p->grow.off_r >= 0) ||
+                                        (p->gen_col && !p->g2c.on && p->gcol.off_r >= 0)))
+        if (int rc = tune_gen_ip(p)) return rc;
     if (p->g2c.avail && !p->g2c.on)
         if (int rc = tune_gen_columns(p)) return rc;
     const double lx = std::log2((double)p->nx), ly = std::log2((double)p->ny);
@@ -1413,7 +1518,7 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     else if (plan->gen_row)
     {
         const int ns = plan->gen_rows;
-        const size_t lds = gen_lds_bytes(plan, (size_t)ns * plan->grow.n);
+        const size_t lds = gen_lds_bytes(plan, plan->grow, (size_t)ns * plan->grow.n);
         const dim3 grid((unsigned)((plan->nx + ns - 1) / ns), (unsigned)plan->batch);
         const unsigned nt = gen_threads(lds, plan->grow);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(plan->gen_tab);

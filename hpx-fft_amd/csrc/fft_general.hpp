@@ -14,6 +14,16 @@
 //     B[(j / Ns) r Ns + k + q Ns] = y_q
 // Radices 2, 3, 4, 5 and 7 run as register butterflies (one LDS pass per
 // stage); larger primes (up to 61) by a generic stage of two passes.
+//
+// Direct plans whose radices are all at most 7 can run in place instead, in
+// one buffer (half the LDS, so twice the workgroups per CU): the loads scatter
+// element e to position pos(e), the mixed-radix digit reversal of e, and
+// decimation-in-time stages r_1, ..., r_S then leave X[k] at position k:
+//   stage with sub-length Ls, butterfly (block c, j < Ls), base = c r Ls + j:
+//     x_q = A[base + q Ls] w_{r Ls}^{q j},  A[base + q Ls] = DFT_r(x)_q;
+//   e = d_S + r_S (d_{S-1} + r_{S-1} (... + r_2 d_1)), d_s < r_s:
+//     pos(e) = sum_s d_s r_1 ... r_{s-1}.
+// The plan's table holds pos() as ints after the twiddles.
 // Twiddles and the roots w_r^e = w_n^{e n/r} come from the plan's exact FP64
 // table (rounded once for FP32); a register butterfly loads w^k and forms
 // w^{qk} (q < r <= 7) as FP64 powers of it.
@@ -44,7 +54,18 @@ struct GenPlan
     int radix[GEN_MAX_STAGES];      // stage radices, product n
     int len;                        // DFT length: n, or the length Bluestein's algorithm maps to n
     int off_c, off_b;               // Bluestein: chirp and kernel-spectrum offsets in the table
+    int off_r;                      // >= 0: the pos() ints at this table offset (radices <= 7); -1: none
+    int ip;                         // 1: in-place stages (needs off_r >= 0); 0: ping-pong Stockham
 };
+
+// sequence buffers a plan's workgroup holds in LDS
+__host__ __device__ inline int gen_nbuf(const GenPlan &g) { return g.ip ? 1 : 2; }
+
+// LDS position of input element e of a sequence
+__device__ __forceinline__ int gen_pos(const GenPlan &g, const cpx<double> *__restrict__ tab, int e)
+{
+    return g.ip ? reinterpret_cast<const int *>(tab + g.off_r)[e] : e;
+}
 
 // x / d for 0 <= x < 2^20 and d >= 1, from inv = 1/d in FP32: the product is
 // within 2^-21 * x of x/d, so the truncated quotient is off by at most one,
@@ -159,6 +180,63 @@ __device__ void gen_stage_reg(const cpx<T> *A, cpx<T> *B, int nseq, int n, int N
     __syncthreads();
 }
 
+// one in-place decimation-in-time stage of radix R (2, 3, 4, 5, 7) over nseq
+// sequences: butterfly (c, j) reads and writes the R positions
+// c R Ls + j + q Ls, so the butterflies of a stage touch disjoint points
+template <typename T, int R, int NTMAX>
+__device__ void gen_stage_ip(cpx<T> *A, int nseq, int n, int Ls, const cpx<double> *__restrict__ tab)
+{
+    const int t = threadIdx.x, m = n / R, nb = nseq * m, tstep = n / (R * Ls);
+    const float inv_m = 1.0f / (float)m, inv_ls = 1.0f / (float)Ls;
+    cpx<T> w[R];
+#pragma unroll
+    for (int e = 1; e < R; ++e) w[e] = tabv<T>(tab, e * m);
+    w[0] = cpx<T>{T(1), T(0)};
+    const int NTH = gen_nth<NTMAX>();
+    for (int b0 = t; b0 < nb; b0 += 2 * NTH)
+    {
+        cpx<T> x[2][R];
+        cpx<double> w1[2];
+        int base[2] = {-1, -1}, jj[2] = {0, 0};
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+        {
+            const int b = b0 + u * NTH;
+            if (b < nb)
+            {
+                const int seq = gdiv(b, m, inv_m), r = b - seq * m;
+                const int c = gdiv(r, Ls, inv_ls), j = r - c * Ls;
+                base[u] = seq * n + c * R * Ls + j;
+#pragma unroll
+                for (int q = 0; q < R; ++q) x[u][q] = A[base[u] + q * Ls];
+                w1[u] = tab[j * tstep];
+                jj[u] = j;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+        {
+            if (base[u] < 0) continue;
+            if (jj[u] > 0)
+            {
+                cpx<double> wq = w1[u];
+                x[u][1] = cmul(x[u][1], cpx<T>{(T)wq.x, (T)wq.y});
+#pragma unroll
+                for (int q = 2; q < R; ++q)
+                {
+                    wq = cmul(wq, w1[u]);
+                    x[u][q] = cmul(x[u][q], cpx<T>{(T)wq.x, (T)wq.y});
+                }
+            }
+            dft_small<T, R>(x[u], w);
+            cpx<T> *p = A + base[u];
+#pragma unroll
+            for (int q = 0; q < R; ++q) p[q * Ls] = x[u][q];
+        }
+    }
+    __syncthreads();
+}
+
 // one stage of a prime radix r > 7: twiddle A in place, then every output
 // reads the r inputs of its butterfly from A and goes to B
 template <typename T, int NTMAX>
@@ -219,6 +297,27 @@ __device__ cpx<T> *gen_stages(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, 
     return A;
 }
 
+// all stages of an in-place plan (radices <= 7) on the digit-reversed
+// sequences in A; the result is in A in natural order
+template <typename T, int NTMAX>
+__device__ void gen_stages_ip(cpx<T> *A, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
+{
+    int Ls = 1;
+    for (int s = 0; s < g.nstages; ++s)
+    {
+        const int r = g.radix[s];
+        switch (r)
+        {
+        case 2: gen_stage_ip<T, 2, NTMAX>(A, nseq, g.n, Ls, tab); break;
+        case 3: gen_stage_ip<T, 3, NTMAX>(A, nseq, g.n, Ls, tab); break;
+        case 4: gen_stage_ip<T, 4, NTMAX>(A, nseq, g.n, Ls, tab); break;
+        case 5: gen_stage_ip<T, 5, NTMAX>(A, nseq, g.n, Ls, tab); break;
+        default: gen_stage_ip<T, 7, NTMAX>(A, nseq, g.n, Ls, tab); break;   // the host admits only 2..7 here
+        }
+        Ls *= r;
+    }
+}
+
 // The DFT of length g.len of nseq sequences held at A[s*g.n + e], e < g.len.
 // Direct plans (g.len == g.n) run the Stockham stages.  Otherwise Bluestein's
 // algorithm: with the chirp c[j] = w_{2 len}^{j^2} (tab + off_c),
@@ -226,10 +325,16 @@ __device__ cpx<T> *gen_stages(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, 
 // a circular convolution of length n >= 2 len - 1, computed as
 // IDFT(DFT(a) * Bh) with Bh = DFT(b) / n (tab + off_b, b[m] = conj c[|m|]);
 // the inverse DFT is conj(DFT(conj .)).  Returns the buffer with X at
-// [s*g.n + k], k < g.len.
+// [s*g.n + k], k < g.len.  In-place plans (g.ip) take A loaded in the
+// gen_pos() order and return A.
 template <typename T, int NTMAX>
 __device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
 {
+    if (g.ip)
+    {
+        gen_stages_ip<T, NTMAX>(A, nseq, g, tab);
+        return A;
+    }
     if (g.len == g.n) return gen_stages<T, NTMAX>(A, B, nseq, g, tab);
     const int t = threadIdx.x, n = g.n, len = g.len, total = nseq * n;
     const cpx<double> *chirp = tab + g.off_c, *bh = tab + g.off_b;
@@ -270,7 +375,7 @@ template <typename T> __device__ __forceinline__ cpx<T> *gen_lds()
 //   X[k] = Fe + w_{Ny}^k Fo, X[L-k] = conj(Fe - w_{Ny}^k Fo),
 //   Fe = (Z[k] + conj Z[L-k]) / 2, Fo = (Z[k] - conj Z[L-k]) / (2i).
 // tabL: the row plan's table; tab2: w_{Ny}^m (m < Ny).  blockIdx.x = row group
-// (XCD order), .y = grid.  Dynamic LDS: 2 * nseq * g.n complex, then nseq
+// (XCD order), .y = grid.  Dynamic LDS: gen_nbuf(g) * nseq * g.n complex, then nseq
 // words.  16-bit storage S (T = float): the input row is unscaled by
 // 2^-in_exp[row] when given, and the output row is stored times 2^e with
 // e = pick_exp(its largest component), written to out_exp[row], as the
@@ -282,7 +387,7 @@ __global__ void __launch_bounds__(NTMAX) gen_row_r2c_kernel(RowArgs a, GenPlan g
     constexpr bool S16 = is16<S>();
     cpx<T> *buf = gen_lds<T>();
     const int t = threadIdx.x, L = g.len, ST = g.n;
-    unsigned *smax = reinterpret_cast<unsigned *>(buf + 2 * nseq * ST);
+    unsigned *smax = reinterpret_cast<unsigned *>(buf + gen_nbuf(g) * nseq * ST);
     const long long b = blockIdx.y;
     const int row0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * nseq;
     const int nrow = min(nseq, a.nrows - row0);
@@ -305,7 +410,7 @@ __global__ void __launch_bounds__(NTMAX) gen_row_r2c_kernel(RowArgs a, GenPlan g
                 if (a.in_exp) z = scal(z, (T)ldexpf(1.0f, -a.in_exp[b * a.exp_dist + row0 + s]));
             }
         }
-        buf[s * ST + e] = z;
+        buf[s * ST + gen_pos(g, tabL, e)] = z;
     }
     __syncthreads();
     const cpx<T> *Zs = gen_fft<T, NTMAX>(buf, buf + nseq * ST, nseq, g, tabL);
@@ -379,7 +484,7 @@ __global__ void __launch_bounds__(NTMAX) gen_row_r2c_kernel(RowArgs a, GenPlan g
 // forward transform, in place).  CONJ: the inverse transform, computed as
 // conj(DFT(conj .)).  blockIdx.x = strip (XCD order: neighbouring strips,
 // which share 128-byte lines, run behind one L2), .y = grid.
-// tab: the column plan's table.  Dynamic LDS: 2 * W * g.n complex, then W
+// tab: the column plan's table.  Dynamic LDS: gen_nbuf(g) * W * g.n complex, then W
 // words.  16-bit storage S (T = float, forward only): rows are unscaled by
 // 2^-row_exp[row] on load, and column c is stored times 2^e with e =
 // pick_exp(its largest component), written to out_exp[c].
@@ -389,7 +494,7 @@ __global__ void __launch_bounds__(NTMAX) gen_col_kernel(ColArgs a, GenPlan g, co
     constexpr bool S16 = is16<S>();
     cpx<T> *buf = gen_lds<T>();
     const int t = threadIdx.x, n = g.len, ST = g.n;
-    unsigned *smax = reinterpret_cast<unsigned *>(buf + 2 * W * ST);
+    unsigned *smax = reinterpret_cast<unsigned *>(buf + gen_nbuf(g) * W * ST);
     const long long b = blockIdx.y;
     const int col0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * W;
     const int nvalid = min(W, a.ncols - col0);
@@ -413,7 +518,7 @@ __global__ void __launch_bounds__(NTMAX) gen_col_kernel(ColArgs a, GenPlan g, co
                 if (a.row_exp) z = scal(z, (T)ldexpf(1.0f, -a.row_exp[b * a.rexp_dist + e]));
             }
         }
-        buf[w * ST + e] = CONJ ? conjc(z) : z;
+        buf[w * ST + gen_pos(g, tab, e)] = CONJ ? conjc(z) : z;
     }
     __syncthreads();
     const cpx<T> *X = gen_fft<T, NTMAX>(buf, buf + ST * W, W, g, tab);
@@ -443,7 +548,7 @@ __global__ void __launch_bounds__(NTMAX) gen_col_kernel(ColArgs a, GenPlan g, co
 
 // Row c2r of any even Ny (the row half of the adjoint when HALVE, the
 // unnormalised inverse c2r otherwise): half-spectrum row G[0..L] in, Ny reals
-// out, nseq rows per workgroup (dynamic LDS: 2 * nseq * g.n complex),
+// out, nseq rows per workgroup (dynamic LDS: gen_nbuf(g) * nseq * g.n complex),
 //   H = G on 0<k<L (halved when HALVE), H[0] = Re G[0], H[L] = Re G[L],
 //   Z[k] = (H[k] + conj H[L-k]) + i (H[k] - conj H[L-k]) conj(w_{Ny}^k),
 //   out[2n] + i out[2n+1] = IDFT_L(Z) = conj(DFT(conj Z)).
@@ -481,7 +586,7 @@ __global__ void __launch_bounds__(NTMAX) gen_row_c2r_kernel(RowArgs a, GenPlan g
                 z = cpx<T>{s0.x - d0.y, s0.y + d0.x};
             }
         }
-        buf[s * ST + k] = conjc(z);
+        buf[s * ST + gen_pos(g, tabL, k)] = conjc(z);
     }
     __syncthreads();
     const cpx<T> *U = gen_fft<T, NTMAX>(buf, buf + nseq * ST, nseq, g, tabL);
@@ -577,7 +682,7 @@ __global__ void __launch_bounds__(NTMAX) gen_strided_kernel(GenStride a, GenPlan
             }
             if (a.conj_in) z = conjc(z);
         }
-        buf[w * ST + e] = z;
+        buf[w * ST + gen_pos(g, tab, e)] = z;
     }
     __syncthreads();
     const cpx<T> *X = gen_fft<T, NTMAX>(buf, buf + ST * W, W, g, tab);

@@ -134,6 +134,39 @@ def test_measure_times_one_and_two_level_columns(gpu, nx, ny, prec, tol):
     assert oracle.rel_l2(got, oracle.r2c_2d(x, ny)) <= tol
 
 
+@pytest.mark.parametrize("nx,ny,prec,tol", [(3000, 3000, "f64", 1e-12), (1500, 6000, "f32", 1e-5),
+                                             (840, 6300, "f64", 1e-12), (1000, 1000, "f64", 1e-12),
+                                             (343, 2 * 2401, "f64", 1e-12)])
+def test_in_place_and_ping_pong_stages(gpu, nx, ny, prec, tol):
+    """Direct general plans whose radices are all at most 7 run ping-pong
+    Stockham stages or in-place decimation-in-time stages (inputs scattered
+    to their digit-reversed positions).  The estimate runs in place; measure
+    times both forms of each one-level pass and keeps the faster.  Either
+    way the result is the oracle's."""
+    import torch
+    import hpxfft_amd
+    x = oracle.fill(nx, ny, seed=7 * nx + ny, dist="upm1")
+    ref = oracle.r2c_2d(x, ny)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    for effort in ("estimate", "measure"):
+        print(f"stages {prec} {nx}x{ny} {effort}", flush=True)
+        d = torch.from_numpy(x).to(device="cuda", dtype=dt)
+        p = hpxfft_amd.Plan(nx, ny, precision=prec, effort=effort)
+        desc = p.describe()
+        if effort == "measure":
+            assert "row pass autotuned (ms): ping-pong" in desc, desc
+            assert "column pass autotuned (ms): ping-pong" in desc, desc
+        else:
+            assert "Stockham" not in desc and desc.count("in-place decimation") == 2, desc
+        p.exec(d.data_ptr(), d.data_ptr())
+        torch.cuda.synchronize()
+        p.close()
+        got = d.cpu().numpy().astype(np.float64)[:, : 2 * (ny // 2 + 1)].copy().view(np.complex128)
+        err = oracle.rel_l2(got, ref)
+        print(f"  rel-L2 = {err:.3e}", flush=True)
+        assert err <= tol, err
+
+
 def test_two_level_fp32_bluestein_phase(gpu):
     """FP32 Bluestein FFTs of up to 8192 points: 8186 = 2 * 4093 (prime) as a
     2-point phase and a 4093-point Bluestein phase."""
