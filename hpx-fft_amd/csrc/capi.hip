@@ -393,6 +393,7 @@ bool gen_setup(size_t n, size_t cap, hpxfft_dev::GenPlan &g)
         bool small = true;
         for (int i = 0; i < g.nstages; ++i) small = small && g.radix[i] <= 7;
         g.off_r = small ? g.n : -1;
+        g.off_bp = -1;
         g.ip = small ? 1 : 0;
         return true;
     }
@@ -402,10 +403,16 @@ bool gen_setup(size_t n, size_t cap, hpxfft_dev::GenPlan &g)
     g.len = (int)n;
     g.off_c = (int)M;
     g.off_b = (int)(M + n);
+    // the power-of-two FFT (radices 4 and 2) runs in place by default: the
+    // kernel spectrum also in pos() order
     g.off_r = -1;
-    g.ip = 0;
+    g.off_bp = (int)(2 * M + n);
+    g.ip = 1;
     return true;
 }
+
+// whether a plan can run its stages in place
+bool gen_can_ip(const hpxfft_dev::GenPlan &g) { return g.off_r >= 0 || g.off_bp >= 0; }
 
 
 // two-level split N = P * Q (N <= 2^20): both sub-lengths within one
@@ -451,11 +458,12 @@ int gen_width(const hpxfft_dev::GenPlan &g, size_t cap, size_t wmax)
 }
 
 // table entries (complex FP64) of a general plan: w_n^m (m < n); for
-// Bluestein the chirp (len) and the kernel spectrum (n); for in-place plans
-// the n int input positions, four per entry
+// Bluestein the chirp (len), the kernel spectrum (n) and the same spectrum in
+// the in-place stages' pos() order (n); for other in-place plans the n int
+// input positions, four per entry
 size_t gen_table_size(const hpxfft_dev::GenPlan &g)
 {
-    if (g.len != g.n) return (size_t)(2 * g.n + g.len);
+    if (g.len != g.n) return (size_t)(3 * g.n + g.len);
     return (size_t)g.n + (g.off_r >= 0 ? ((size_t)g.n + 3) / 4 : 0);
 }
 
@@ -549,8 +557,9 @@ void gen_table(const hpxfft_dev::GenPlan &g, std::vector<double> &h, size_t off)
     }
     for (uint64_t m = 0; m < n; ++m)
     {
-        h[2 * (off + n + len + m)] = (double)(br[m] / (long double)n);
-        h[2 * (off + n + len + m) + 1] = (double)(bi[m] / (long double)n);
+        const size_t pm = (size_t)gen_digit_rev(g, (int)m);
+        h[2 * (off + n + len + m)] = h[2 * (off + (size_t)g.off_bp + pm)] = (double)(br[m] / (long double)n);
+        h[2 * (off + n + len + m) + 1] = h[2 * (off + (size_t)g.off_bp + pm) + 1] = (double)(bi[m] / (long double)n);
     }
 }
 
@@ -563,10 +572,11 @@ std::string radices(const hpxfft_dev::GenPlan &g)
 
 std::string gen_method(const hpxfft_dev::GenPlan &g)
 {
-    if (g.ip) return "mixed-radix in-place decimation in time in LDS, radices " + radices(g);
-    if (g.len == g.n) return "mixed-radix Stockham in LDS, radices " + radices(g);
+    if (g.len == g.n)
+        return std::string(g.ip ? "mixed-radix in-place decimation in time" : "mixed-radix Stockham") +
+               " in LDS, radices " + radices(g);
     return "Bluestein (chirp convolution through a " + std::to_string(g.n) + "-point FFT in LDS, radices " +
-           radices(g) + ")";
+           radices(g) + (g.ip ? ", in place: decimation in frequency, then in time)" : ", Stockham)");
 }
 
 std::string describe(const hpxfft_plan_s *p)
@@ -1053,8 +1063,8 @@ int tune_gen_ip(hpxfft_plan_s *p)
         return HPXFFT_OK;
     }
     const int ip_row = p->grow.ip, ip_col = p->gcol.ip;
-    const bool row_ok = p->gen_row && !p->g2r.on && p->grow.off_r >= 0;
-    const bool col_ok = p->gen_col && !p->g2c.on && p->gcol.off_r >= 0;
+    const bool row_ok = p->gen_row && !p->g2r.on && gen_can_ip(p->grow);
+    const bool col_ok = p->gen_col && !p->g2c.on && gen_can_ip(p->gcol);
     int rc = HPXFFT_OK;
     hipError_t e = hipMemsetAsync(in, 0, bin, p->stream);
     for (int ip = 0; ip < 2 && e == hipSuccess && rc == HPXFFT_OK; ++ip)
@@ -1303,8 +1313,8 @@ int plan_common(hpxfft_plan_s *p)
     if (int rc = alloc_events(p)) return rc;
     if (p->four_step && p->effort >= HPXFFT_MEASURE)
         if (int rc = tune_split(p)) return rc;
-    if (p->effort >= HPXFFT_MEASURE && ((p->gen_row && !p->g2r.on && p->grow.off_r >= 0) ||
-                                        (p->gen_col && !p->g2c.on && p->gcol.off_r >= 0)))
+    if (p->effort >= HPXFFT_MEASURE && ((p->gen_row && !p->g2r.on && gen_can_ip(p->grow)) ||
+                                        (p->gen_col && !p->g2c.on && gen_can_ip(p->gcol))))
         if (int rc = tune_gen_ip(p)) return rc;
     if (p->g2c.avail && !p->g2c.on)
         if (int rc = tune_gen_columns(p)) return rc;

@@ -23,7 +23,11 @@
 //     x_q = A[base + q Ls] w_{r Ls}^{q j},  A[base + q Ls] = DFT_r(x)_q;
 //   e = d_S + r_S (d_{S-1} + r_{S-1} (... + r_2 d_1)), d_s < r_s:
 //     pos(e) = sum_s d_s r_1 ... r_{s-1}.
-// The plan's table holds pos() as ints after the twiddles.
+// The plan's table holds pos() as ints after the twiddles.  Bluestein plans
+// run in place too: decimation-in-frequency stages r_S, ..., r_1 take the
+// natural order to X[k] at pos(k), the product with the kernel spectrum
+// reads that spectrum stored in the same order, and the decimation-in-time
+// stages bring the inverse transform back to natural order.
 // Twiddles and the roots w_r^e = w_n^{e n/r} come from the plan's exact FP64
 // table (rounded once for FP32); a register butterfly loads w^k and forms
 // w^{qk} (q < r <= 7) as FP64 powers of it.
@@ -55,7 +59,8 @@ struct GenPlan
     int len;                        // DFT length: n, or the length Bluestein's algorithm maps to n
     int off_c, off_b;               // Bluestein: chirp and kernel-spectrum offsets in the table
     int off_r;                      // >= 0: the pos() ints at this table offset (radices <= 7); -1: none
-    int ip;                         // 1: in-place stages (needs off_r >= 0); 0: ping-pong Stockham
+    int off_bp;                     // Bluestein: the kernel spectrum in pos() order at this offset
+    int ip;                         // 1: in-place stages (needs off_r >= 0, or Bluestein); 0: ping-pong
 };
 
 // sequence buffers a plan's workgroup holds in LDS
@@ -64,7 +69,7 @@ __host__ __device__ inline int gen_nbuf(const GenPlan &g) { return g.ip ? 1 : 2;
 // LDS position of input element e of a sequence
 __device__ __forceinline__ int gen_pos(const GenPlan &g, const cpx<double> *__restrict__ tab, int e)
 {
-    return g.ip ? reinterpret_cast<const int *>(tab + g.off_r)[e] : e;
+    return g.ip && g.off_r >= 0 ? reinterpret_cast<const int *>(tab + g.off_r)[e] : e;
 }
 
 // x / d for 0 <= x < 2^20 and d >= 1, from inv = 1/d in FP32: the product is
@@ -297,6 +302,84 @@ __device__ cpx<T> *gen_stages(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, 
     return A;
 }
 
+// one in-place decimation-in-frequency stage of radix R on blocks of Lb
+// points: butterfly (c, j < Lb/R) reads the R positions c Lb + j + q Lb/R,
+// transforms them and multiplies output p by w_{Lb}^{j p}
+template <typename T, int R, int NTMAX>
+__device__ void gen_stage_dif(cpx<T> *A, int nseq, int n, int Lb, const cpx<double> *__restrict__ tab)
+{
+    const int t = threadIdx.x, per = n / R, nb = nseq * per, m = Lb / R, tstep = n / Lb;
+    const float inv_per = 1.0f / (float)per, inv_m = 1.0f / (float)m;
+    cpx<T> w[R];
+#pragma unroll
+    for (int e = 1; e < R; ++e) w[e] = tabv<T>(tab, e * per);
+    w[0] = cpx<T>{T(1), T(0)};
+    const int NTH = gen_nth<NTMAX>();
+    for (int b0 = t; b0 < nb; b0 += 2 * NTH)
+    {
+        cpx<T> x[2][R];
+        cpx<double> w1[2];
+        int base[2] = {-1, -1}, jj[2] = {0, 0};
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+        {
+            const int b = b0 + u * NTH;
+            if (b < nb)
+            {
+                const int seq = gdiv(b, per, inv_per), r = b - seq * per;
+                const int c = gdiv(r, m, inv_m), j = r - c * m;
+                base[u] = seq * n + c * Lb + j;
+#pragma unroll
+                for (int q = 0; q < R; ++q) x[u][q] = A[base[u] + q * m];
+                w1[u] = tab[j * tstep];
+                jj[u] = j;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+        {
+            if (base[u] < 0) continue;
+            dft_small<T, R>(x[u], w);
+            if (jj[u] > 0)
+            {
+                cpx<double> wq = w1[u];
+                x[u][1] = cmul(x[u][1], cpx<T>{(T)wq.x, (T)wq.y});
+#pragma unroll
+                for (int q = 2; q < R; ++q)
+                {
+                    wq = cmul(wq, w1[u]);
+                    x[u][q] = cmul(x[u][q], cpx<T>{(T)wq.x, (T)wq.y});
+                }
+            }
+            cpx<T> *p = A + base[u];
+#pragma unroll
+            for (int q = 0; q < R; ++q) p[q * m] = x[u][q];
+        }
+    }
+    __syncthreads();
+}
+
+// all decimation-in-frequency stages (r_S first) of a plan whose radices are
+// at most 7: natural order in A to X[k] at pos(k)
+template <typename T, int NTMAX>
+__device__ void gen_stages_dif(cpx<T> *A, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
+{
+    int Lb = g.n;
+    for (int s = g.nstages - 1; s >= 0; --s)
+    {
+        const int r = g.radix[s];
+        switch (r)
+        {
+        case 2: gen_stage_dif<T, 2, NTMAX>(A, nseq, g.n, Lb, tab); break;
+        case 3: gen_stage_dif<T, 3, NTMAX>(A, nseq, g.n, Lb, tab); break;
+        case 4: gen_stage_dif<T, 4, NTMAX>(A, nseq, g.n, Lb, tab); break;
+        case 5: gen_stage_dif<T, 5, NTMAX>(A, nseq, g.n, Lb, tab); break;
+        default: gen_stage_dif<T, 7, NTMAX>(A, nseq, g.n, Lb, tab); break;   // the host admits only 2..7 here
+        }
+        Lb /= r;
+    }
+}
+
 // all stages of an in-place plan (radices <= 7) on the digit-reversed
 // sequences in A; the result is in A in natural order
 template <typename T, int NTMAX>
@@ -330,7 +413,7 @@ __device__ void gen_stages_ip(cpx<T> *A, int nseq, const GenPlan &g, const cpx<d
 template <typename T, int NTMAX>
 __device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
 {
-    if (g.ip)
+    if (g.ip && g.len == g.n)
     {
         gen_stages_ip<T, NTMAX>(A, nseq, g, tab);
         return A;
@@ -345,6 +428,27 @@ __device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, con
         A[i] = e < len ? cmul(A[i], tabv<T>(chirp, e)) : cpx<T>{T(0), T(0)};
     }
     __syncthreads();
+    if (g.ip)
+    {
+        // in place: DIF to the pos() order, the kernel spectrum in that order,
+        // then DIT back to natural order
+        gen_stages_dif<T, NTMAX>(A, nseq, g, tab);
+        const cpx<double> *bhp = tab + g.off_bp;
+        for (int i = t; i < total; i += gen_nth<NTMAX>())
+        {
+            const int p = i - gdiv(i, n, inv_n) * n;
+            A[i] = conjc(cmul(A[i], tabv<T>(bhp, p)));
+        }
+        __syncthreads();
+        gen_stages_ip<T, NTMAX>(A, nseq, g, tab);
+        for (int i = t; i < total; i += gen_nth<NTMAX>())
+        {
+            const int e = i - gdiv(i, n, inv_n) * n;
+            if (e < len) A[i] = cmul(conjc(A[i]), tabv<T>(chirp, e));
+        }
+        __syncthreads();
+        return A;
+    }
     cpx<T> *F = gen_stages<T, NTMAX>(A, B, nseq, g, tab);
     for (int i = t; i < total; i += gen_nth<NTMAX>())
     {

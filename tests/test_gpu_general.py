@@ -136,13 +136,18 @@ def test_measure_times_one_and_two_level_columns(gpu, nx, ny, prec, tol):
 
 @pytest.mark.parametrize("nx,ny,prec,tol", [(3000, 3000, "f64", 1e-12), (1500, 6000, "f32", 1e-5),
                                              (840, 6300, "f64", 1e-12), (1000, 1000, "f64", 1e-12),
-                                             (343, 2 * 2401, "f64", 1e-12)])
+                                             (343, 2 * 2401, "f64", 1e-12),
+                                             # Bluestein rows and columns (primes 1031, 127, 67)
+                                             (1031, 2 * 127, "f64", 1e-12), (67, 2 * 1031, "f32", 1e-5),
+                                             (127, 4096, "f64", 1e-12)])
 def test_in_place_and_ping_pong_stages(gpu, nx, ny, prec, tol):
     """Direct general plans whose radices are all at most 7 run ping-pong
     Stockham stages or in-place decimation-in-time stages (inputs scattered
-    to their digit-reversed positions).  The estimate runs in place; measure
-    times both forms of each one-level pass and keeps the faster.  Either
-    way the result is the oracle's."""
+    to their digit-reversed positions); Bluestein plans run their
+    power-of-two FFTs ping-pong or in place (decimation in frequency, the
+    kernel spectrum in digit-reversed order, decimation in time).  The
+    estimate runs in place; measure times both forms of each one-level pass
+    and keeps the faster.  Either way the result is the oracle's."""
     import torch
     import hpxfft_amd
     x = oracle.fill(nx, ny, seed=7 * nx + ny, dist="upm1")
@@ -153,11 +158,13 @@ def test_in_place_and_ping_pong_stages(gpu, nx, ny, prec, tol):
         d = torch.from_numpy(x).to(device="cuda", dtype=dt)
         p = hpxfft_amd.Plan(nx, ny, precision=prec, effort=effort)
         desc = p.describe()
+        general = [ny // 2 != 1 << (ny // 2).bit_length() - 1, nx != 1 << nx.bit_length() - 1]
         if effort == "measure":
-            assert "row pass autotuned (ms): ping-pong" in desc, desc
-            assert "column pass autotuned (ms): ping-pong" in desc, desc
+            assert ("row pass autotuned (ms): ping-pong" in desc) == general[0], desc
+            assert ("column pass autotuned (ms): ping-pong" in desc) == general[1], desc
         else:
-            assert "Stockham" not in desc and desc.count("in-place decimation") == 2, desc
+            assert "mixed-radix Stockham" not in desc and ", Stockham)" not in desc, desc
+            assert desc.count("in-place decimation") + desc.count("in place: decimation") == sum(general), desc
         p.exec(d.data_ptr(), d.data_ptr())
         torch.cuda.synchronize()
         p.close()
