@@ -414,6 +414,24 @@ bool gen_setup(size_t n, size_t cap, hpxfft_dev::GenPlan &g)
 // whether a plan can run its stages in place
 bool gen_can_ip(const hpxfft_dev::GenPlan &g) { return g.off_r >= 0 || g.off_bp >= 0; }
 
+// the last resort for a one-level length that no other plan covers (a prime
+// factor above 61 that no split avoids): Bluestein's algorithm through an
+// FFT of more than `cap` and at most 2 cap points, held in one LDS buffer
+// and so run in place only
+bool gen_setup_in_place_only(size_t n, size_t cap, hpxfft_dev::GenPlan &g)
+{
+    size_t M = 1;
+    while (M < 2 * n - 1) M *= 2;
+    if (M <= cap || M > 2 * cap || !gen_factor(M, g)) return false;
+    g.len = (int)n;
+    g.off_c = (int)M;
+    g.off_b = (int)(M + n);
+    g.off_r = -1;
+    g.off_bp = (int)(2 * M + n);
+    g.ip = 1;
+    return true;
+}
+
 
 // two-level split N = P * Q (N <= 2^20): both sub-lengths within one
 // workgroup, direct plans preferred over Bluestein, then the most balanced
@@ -1053,6 +1071,11 @@ int tune_split(hpxfft_plan_s *p)
 // If the buffers cannot be allocated the estimate stays.
 int tune_gen_ip(hpxfft_plan_s *p)
 {
+    // both forms must fit: the ping-pong one holds twice the points
+    const size_t cap = p->prec == P_F64 ? hpxfft_dev::gen_max_points<double>() : hpxfft_dev::gen_max_points<float>();
+    const bool row_ok = p->gen_row && !p->g2r.on && gen_can_ip(p->grow) && (size_t)p->gen_rows * p->grow.n <= cap;
+    const bool col_ok = p->gen_col && !p->g2c.on && gen_can_ip(p->gcol) && (size_t)p->gen_w * p->gcol.n <= cap;
+    if (!row_ok && !col_ok) return HPXFFT_OK;
     const size_t sb = storage_bytes(p->prec);
     const size_t bin = p->batch * p->dist_in * sb, bout = p->batch * p->dist_out * 2 * sb;
     void *in = nullptr, *out = nullptr;
@@ -1063,8 +1086,6 @@ int tune_gen_ip(hpxfft_plan_s *p)
         return HPXFFT_OK;
     }
     const int ip_row = p->grow.ip, ip_col = p->gcol.ip;
-    const bool row_ok = p->gen_row && !p->g2r.on && gen_can_ip(p->grow);
-    const bool col_ok = p->gen_col && !p->g2c.on && gen_can_ip(p->gcol);
     int rc = HPXFFT_OK;
     hipError_t e = hipMemsetAsync(in, 0, bin, p->stream);
     for (int ip = 0; ip < 2 && e == hipSuccess && rc == HPXFFT_OK; ++ip)
@@ -1171,34 +1192,46 @@ int plan_common(hpxfft_plan_s *p)
         auto why = [&](const char *what, size_t n) {
             return std::string(what) + " = " + std::to_string(n) + " needs at most " + std::to_string(cap) +
                    " points with prime factors <= " + std::to_string(hpxfft_dev::GEN_MAX_RADIX) +
-                   ", or a Bluestein FFT (the power of two >= 2n-1) of at most " + std::to_string(cap) +
-                   " points, or (up to 2^20 points) a split n = P*Q into two such lengths, for this precision (got " +
-                   dims + ")";
+                   ", or a Bluestein FFT (the power of two >= 2n-1) of at most " + std::to_string(2 * cap) +
+                   " points, or (up to 2^20 points) a split n = P*Q into two lengths of at most " +
+                   std::to_string(cap) + " points, each with such prime factors or a Bluestein FFT of at most " +
+                   std::to_string(cap) + " points, for this precision (got " + dims + ")";
         };
         const size_t wmax = 128 / (2 * storage_bytes(p->prec));   // one 128-byte segment
         if (p->gen_row && !gen_setup(p->ny / 2, cap, p->grow))
         {
-            if (s16)
-                return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two only "
-                                                     "within one workgroup: F64 or F32 for row length Ny/2 = " +
-                                                     std::to_string(p->ny / 2) + " (got " + dims + ")");
-            if (!gen2_setup(p->ny / 2, cap, p->g2r))
+            if (!s16 && gen2_setup(p->ny / 2, cap, p->g2r))
+            {
+                if (p->nx > 65535 || p->batch > 65535)
+                    return fail(HPXFFT_UNSUPPORTED_SIZE,
+                                "two-level rows take at most 65535 rows and grids (got " + dims + ")");
+                p->g2r.WA = std::min(gen_width(p->g2r.A, cap, wmax), p->g2r.P);
+                p->g2r.WB = std::min(gen_width(p->g2r.B, cap, wmax), p->g2r.Q);
+            }
+            else if (!gen_setup_in_place_only(p->ny / 2, cap, p->grow))
+            {
+                if (s16)
+                    return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two "
+                                                         "only within one workgroup: F64 or F32 for row length "
+                                                         "Ny/2 = " + std::to_string(p->ny / 2) + " (got " + dims + ")");
                 return fail(HPXFFT_UNSUPPORTED_SIZE, why("row length Ny/2", p->ny / 2));
-            if (p->nx > 65535 || p->batch > 65535)
-                return fail(HPXFFT_UNSUPPORTED_SIZE, "two-level rows take at most 65535 rows and grids (got " + dims + ")");
-            p->g2r.WA = std::min(gen_width(p->g2r.A, cap, wmax), p->g2r.P);
-            p->g2r.WB = std::min(gen_width(p->g2r.B, cap, wmax), p->g2r.Q);
+            }
         }
         if (p->gen_col && !gen_setup(p->nx, cap, p->gcol))
         {
-            if (s16)
-                return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two only "
-                                                     "within one workgroup: F64 or F32 for column length Nx = " +
-                                                     std::to_string(p->nx) + " (got " + dims + ")");
-            if (!gen2_setup(p->nx, cap, p->g2c))
+            if (!s16 && gen2_setup(p->nx, cap, p->g2c))
+            {
+                p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
+                p->g2c.WB = gen_width(p->g2c.B, cap, wmax);
+            }
+            else if (!gen_setup_in_place_only(p->nx, cap, p->gcol))
+            {
+                if (s16)
+                    return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two "
+                                                         "only within one workgroup: F64 or F32 for column length "
+                                                         "Nx = " + std::to_string(p->nx) + " (got " + dims + ")");
                 return fail(HPXFFT_UNSUPPORTED_SIZE, why("column length Nx", p->nx));
-            p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
-            p->g2c.WB = gen_width(p->g2c.B, cap, wmax);
+            }
         }
         if (p->gen_row && !p->g2r.on)
         {

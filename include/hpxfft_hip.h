@@ -40,10 +40,11 @@ typedef enum
     HPXFFT_UNSUPPORTED_SIZE = 2, /* size outside what the kernels cover: powers of two (Nx <= 2^16,
                                     Ny <= 2^15), or for F64/F32 plans any Nx and even Ny whose
                                     lengths (Nx, Ny/2) have prime factors <= 61 within 4096 (F64) /
-                                    8192 (F32) points, or otherwise up to 2048 (F64) / 4096 (F32)
+                                    8192 (F32) points, or otherwise up to 4096 (F64) / 8192 (F32)
                                     points by Bluestein's algorithm, or up to 2^20 points as a
-                                    product of two such lengths (F16/BF16 storage: the one-level
-                                    lengths, forward) */
+                                    product of two lengths each within 4096 / 8192 points with
+                                    such prime factors or within 2048 / 4096 points by Bluestein
+                                    (F16/BF16 storage: the one-level lengths, forward) */
     HPXFFT_NO_DEVICE = 3,        /* no HIP device visible */
     HPXFFT_OUT_OF_MEMORY = 4,
     HPXFFT_HIP_ERROR = 5,

@@ -76,18 +76,20 @@ def plan_status(nx, ny, batch=1, ld_in=None, dist_in=None, ld_out=None, dist_out
                                    ((1 << 20) + 2, 4)])
 def test_unsupported_sizes(nx, ny):
     """Odd Ny, zero sizes, and FP64 lengths no plan covers: a prime factor
-    above 61 whose Bluestein FFT (the power of two >= 2n-1) exceeds 4096
-    points and that no split n = P*Q avoids (4099 is prime; 8193 = 3 * 2731),
-    or more than 2^20 points (Nx = 2^21, Ny/2 = 2^21)."""
+    above 61 whose Bluestein FFT (the power of two >= 2n-1) exceeds 8192
+    points and that no split n = P*Q into lengths of at most 4096 points
+    (Bluestein FFTs of at most 4096) avoids (4099 is prime; 8193 = 3 * 2731,
+    and 2731 needs an 8192-point FFT), or more than 2^20 points (Nx = 2^21,
+    Ny/2 = 2^21)."""
     assert plan_status(nx, ny) == _lib.UNSUPPORTED_SIZE
     assert hpxfft_amd.last_error()
 
 
 def test_fp32_bluestein_reaches_twice_as_far():
-    """FP32 holds 8192 points per buffer: Ny/2 = 4093 (prime) fits its
-    Bluestein FFT, which FP64 (4096 points) rejects."""
-    assert plan_status(16, 2 * 4093) == _lib.UNSUPPORTED_SIZE
-    st = plan_status(16, 2 * 4093, prec=1)
+    """FP32 holds twice the points per buffer: Ny/2 = 8191 (prime) fits its
+    16384-point in-place Bluestein FFT, which FP64 (8192 points) rejects."""
+    assert plan_status(16, 2 * 8191) == _lib.UNSUPPORTED_SIZE
+    st = plan_status(16, 2 * 8191, prec=1)
     if st == _lib.OK:
         pytest.skip("a HIP device is present")
     assert st == _lib.NO_DEVICE, hpxfft_amd.last_error()
@@ -96,9 +98,10 @@ def test_fp32_bluestein_reaches_twice_as_far():
 @pytest.mark.parametrize("prec", [2, 3])
 def test_general_lengths_16bit_within_one_workgroup(prec):
     """16-bit storage takes lengths that are not powers of two within one
-    workgroup (here 8 x 14 and a Bluestein length); lengths that need two
-    levels ask for an F64 or F32 plan."""
-    for nx, ny in [(8, 14), (67, 16), (16, 134)]:
+    workgroup (here 8 x 14, Bluestein lengths, and 8191, whose Bluestein FFT
+    runs in place only); lengths that need two levels ask for an F64 or F32
+    plan."""
+    for nx, ny in [(8, 14), (67, 16), (16, 134), (8191, 8), (8, 2 * 8191)]:
         st = plan_status(nx, ny, prec=prec)
         if st == _lib.OK:
             pytest.skip("a HIP device is present")
@@ -110,12 +113,14 @@ def test_general_lengths_16bit_within_one_workgroup(prec):
 
 @pytest.mark.parametrize("nx,ny", [(8, 14), (3, 16), (1000, 64), (64, 1000), (4095, 2),
                                    (16, 134), (67, 16), (2039, 2 * 1031),
-                                   (6000, 8), (16, 2 * 6000), (1 << 17, 4), (4, 1 << 17), (3 * 2039, 2)])
+                                   (6000, 8), (16, 2 * 6000), (1 << 17, 4), (4, 1 << 17), (3 * 2039, 2),
+                                   (4093, 8), (8, 2 * 4093)])
 def test_general_lengths_pass_validation(nx, ny):
     """Lengths that are not powers of two are accepted: prime factors above 61
-    through Bluestein's algorithm, lengths beyond one workgroup as two levels
-    N = P*Q, as are powers of two beyond the power-of-two kernels (Nx > 2^16,
-    Ny > 2^15) (without a device the plan then stops at the device check)."""
+    through Bluestein's algorithm (4093: an 8192-point FFT, in place only),
+    lengths beyond one workgroup as two levels N = P*Q, as are powers of two
+    beyond the power-of-two kernels (Nx > 2^16, Ny > 2^15) (without a device
+    the plan then stops at the device check)."""
     st = plan_status(nx, ny)
     if st == _lib.OK:
         pytest.skip("a HIP device is present")

@@ -35,7 +35,8 @@ SIZES = [
     (8, 134), (67, 16), (67, 134), (127, 262), (1031, 16),
     (4096, 134),    # Bluestein rows, four-step columns
     (67, 4096),     # Bluestein columns, long power-of-two rows
-    (3, 2 * 2039),  # Bluestein rows through a 4096-point FFT (the FP64 limit)
+    (3, 2 * 2039),  # Bluestein rows through a 4096-point FFT (the FP64 ping-pong limit)
+    (4093, 8), (6, 2 * 4093),   # FP64: an 8192-point Bluestein FFT, in place only
 ]
 
 
@@ -174,6 +175,30 @@ def test_in_place_and_ping_pong_stages(gpu, nx, ny, prec, tol):
         assert err <= tol, err
 
 
+@pytest.mark.parametrize("nx,ny", [(8191, 4), (4, 2 * 8191)])
+def test_fp32_in_place_only_bluestein(gpu, nx, ny):
+    """8191 (prime) in FP32: a 16384-point Bluestein FFT, which only the
+    in-place stages hold in one workgroup's LDS; both efforts (measure has no
+    ping-pong form to time against)."""
+    import torch
+    import hpxfft_amd
+    x = oracle.fill(nx, ny, seed=nx + 3 * ny, dist="upm1")
+    ref = oracle.r2c_2d(x, ny)
+    for effort in ("estimate", "measure"):
+        d = torch.from_numpy(x).to(device="cuda", dtype=torch.float32)
+        p = hpxfft_amd.Plan(nx, ny, precision="f32", effort=effort)
+        desc = p.describe()
+        assert "16384-point FFT" in desc and "in place: decimation" in desc, desc
+        assert "autotuned (ms): ping-pong" not in desc, desc
+        p.exec(d.data_ptr(), d.data_ptr())
+        torch.cuda.synchronize()
+        p.close()
+        got = d.cpu().numpy().astype(np.float64)[:, : 2 * (ny // 2 + 1)].copy().view(np.complex128)
+        err = oracle.rel_l2(got, ref)
+        print(f"f32 {nx}x{ny} {effort}: rel-L2 = {err:.3e}", flush=True)
+        assert err <= 1e-5, err
+
+
 def test_two_level_fp32_bluestein_phase(gpu):
     """FP32 Bluestein FFTs of up to 8192 points: 8186 = 2 * 4093 (prime) as a
     2-point phase and a 4093-point Bluestein phase."""
@@ -230,7 +255,7 @@ def test_general_batched(gpu):
 
 
 BACK_SIZES = [(8, 14), (5, 6), (1, 6), (12, 100), (1000, 64), (64, 1000), (4096, 30), (10, 4096), (3, 2),
-              (67, 134), (127, 262), (4096, 134)]
+              (67, 134), (127, 262), (4096, 134), (2053, 4), (4, 2 * 2053)]   # 2053: in place only in FP64
 
 
 @pytest.mark.parametrize("nx,ny", BACK_SIZES)

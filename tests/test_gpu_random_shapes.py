@@ -7,8 +7,8 @@ Bluestein's algorithm and the two-level path all meet.  For each shape the
 plan must either be created, and then match the oracle (FP64 <= 1e-12,
 FP32 <= 1e-5 relative L2, forward; the adjoint on every fourth shape), or
 fail with UNSUPPORTED_SIZE exactly when no plan covers a length: a prime
-factor above 61 whose Bluestein FFT exceeds one workgroup's buffer and that
-no split N = P*Q avoids.
+factor above 61 that no split N = P*Q avoids and whose Bluestein FFT
+exceeds one workgroup's LDS even in place.
 """
 import numpy as np
 import pytest
@@ -53,8 +53,13 @@ def _largest_prime(n):
     return max(big, m)
 
 
+def _bluestein_fft(n):
+    return 1 << (2 * n - 2).bit_length()     # the power of two >= 2n - 1
+
+
 def _one_level(n, cap):
-    return (n <= cap and _largest_prime(n) <= 61) or (1 << (2 * n - 2).bit_length()) <= cap
+    """Within one workgroup's pair of buffers (cap points)."""
+    return (n <= cap and _largest_prime(n) <= 61) or _bluestein_fft(n) <= cap
 
 
 def _supported(n, cap, pow2_max):
@@ -62,10 +67,10 @@ def _supported(n, cap, pow2_max):
         return True
     if _one_level(n, cap):
         return True
-    if n > 1 << 20:
-        return False
-    return any(n % p == 0 and n // p >= 2 and _one_level(p, cap) and _one_level(n // p, cap)
-               for p in range(2, min(cap, n // 2) + 1))
+    if n <= 1 << 20 and any(n % p == 0 and n // p >= 2 and _one_level(p, cap) and _one_level(n // p, cap)
+                            for p in range(2, min(cap, n // 2) + 1)):
+        return True
+    return _bluestein_fft(n) <= 2 * cap      # in place, one buffer of 2 cap points
 
 
 @pytest.mark.parametrize("nx,ny", SHAPES)
