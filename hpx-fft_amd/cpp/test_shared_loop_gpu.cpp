@@ -129,7 +129,8 @@ static int run_all()
         hpxfft::shared::loop f;
         try
         {
-            // Ny = 8198: Ny/2 = 4099 is prime, and its Bluestein FFT (8192 points) exceeds FP64's 4096
+            // Ny = 8198: Ny/2 = 4099 is prime, and its Bluestein FFT (16384 points) exceeds the
+            // 8192 FP64 points one workgroup holds even in place
             f.initialize(hpxfft::shared::vector_2d(8, 8200, 1.0), "estimate");
         }
         catch (const std::invalid_argument &)
