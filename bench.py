@@ -278,8 +278,9 @@ def _split_line(p) -> list:
 def bench_general(nx: int = 3000, ny: int = 3000, reps: int = 10) -> dict:
     """Lengths that are not powers of two, FP64, on the general kernels
     (2-pass byte model): by default 3000 x 3000 (radices 4, 2, 3, 5); with
-    prime lengths such as 2039 x 4078 the Bluestein path; with lengths
-    beyond one workgroup such as 6000 x 12000 the two-level path."""
+    prime lengths such as 2039 x 4078 the Bluestein path; 6000 x 12000 runs
+    its long lengths (radices 2..7) in place in one LDS buffer, and lengths
+    beyond that such as 9000 x 18000 take the two-level path."""
     import torch
     import hpxfft_amd
 
@@ -429,7 +430,8 @@ def main() -> int:
         result["general_length"] = bench_general()
         progress("general lengths: smooth done")
         result["general_length_bluestein"] = bench_general(2039, 2 * 2039)
-        result["general_length_two_level"] = bench_general(6000, 12000)
+        result["general_length_long"] = bench_general(6000, 12000)
+        result["general_length_two_level"] = bench_general(9000, 18000)
         progress("general length done")
         torch.cuda.empty_cache()
     if not args.no_cpu:

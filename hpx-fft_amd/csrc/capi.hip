@@ -376,11 +376,21 @@ bool gen_factor(size_t n, hpxfft_dev::GenPlan &g)
 }
 
 // a general plan for length n: Stockham stages when its prime factors are
-// at most GEN_MAX_RADIX and it fits `cap` points, else Bluestein's algorithm
-// through the power-of-two FFT of length >= 2n - 1 when that fits
-bool gen_setup(size_t n, size_t cap, hpxfft_dev::GenPlan &g)
+// at most GEN_MAX_RADIX and it fits `cap` points (the ping-pong buffers), or
+// in-place stages alone when its radices are at most 7 and it fits `ip_cap`
+// points (one buffer of up to twice as many; measured against two levels:
+// 6000 x 12000 FP64 2.58 -> 1.45 ms, 5000^2 FP64 0.59 -> 0.38 ms), else
+// Bluestein's algorithm through the power-of-two FFT of length >= 2n - 1
+// when that fits `cap`
+bool gen_setup(size_t n, size_t cap, hpxfft_dev::GenPlan &g, size_t ip_cap = 0)
 {
-    if (n <= cap && gen_factor(n, g))
+    bool long_ok = false;
+    if (n > cap && n <= ip_cap && gen_factor(n, g))
+    {
+        long_ok = true;
+        for (int i = 0; i < g.nstages; ++i) long_ok = long_ok && g.radix[i] <= 7;
+    }
+    if ((n <= cap || long_ok) && gen_factor(n, g))
     {
         g.len = (int)n;
         g.off_c = g.off_b = 0;
@@ -1191,14 +1201,14 @@ int plan_common(hpxfft_plan_s *p)
                                             : hpxfft_dev::gen_max_points<float>();
         auto why = [&](const char *what, size_t n) {
             return std::string(what) + " = " + std::to_string(n) + " needs at most " + std::to_string(cap) +
-                   " points with prime factors <= " + std::to_string(hpxfft_dev::GEN_MAX_RADIX) +
-                   ", or a Bluestein FFT (the power of two >= 2n-1) of at most " + std::to_string(2 * cap) +
+                   " points with prime factors <= " + std::to_string(hpxfft_dev::GEN_MAX_RADIX) + " (" +
+                   std::to_string(2 * cap) + " with prime factors <= 7), or a Bluestein FFT (the power of two >= 2n-1) of at most " + std::to_string(2 * cap) +
                    " points, or (up to 2^20 points) a split n = P*Q into two lengths of at most " +
                    std::to_string(cap) + " points, each with such prime factors or a Bluestein FFT of at most " +
                    std::to_string(cap) + " points, for this precision (got " + dims + ")";
         };
         const size_t wmax = 128 / (2 * storage_bytes(p->prec));   // one 128-byte segment
-        if (p->gen_row && !gen_setup(p->ny / 2, cap, p->grow))
+        if (p->gen_row && !gen_setup(p->ny / 2, cap, p->grow, 2 * cap))
         {
             if (!s16 && gen2_setup(p->ny / 2, cap, p->g2r))
             {
@@ -1217,7 +1227,7 @@ int plan_common(hpxfft_plan_s *p)
                 return fail(HPXFFT_UNSUPPORTED_SIZE, why("row length Ny/2", p->ny / 2));
             }
         }
-        if (p->gen_col && !gen_setup(p->nx, cap, p->gcol))
+        if (p->gen_col && !gen_setup(p->nx, cap, p->gcol, 2 * cap))
         {
             if (!s16 && gen2_setup(p->nx, cap, p->g2c))
             {

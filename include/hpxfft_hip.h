@@ -40,7 +40,8 @@ typedef enum
     HPXFFT_UNSUPPORTED_SIZE = 2, /* size outside what the kernels cover: powers of two (Nx <= 2^16,
                                     Ny <= 2^15), or for F64/F32 plans any Nx and even Ny whose
                                     lengths (Nx, Ny/2) have prime factors <= 61 within 4096 (F64) /
-                                    8192 (F32) points, or otherwise up to 4096 (F64) / 8192 (F32)
+                                    8192 (F32) points (prime factors <= 7: within 8192 / 16384
+                                    points), or otherwise up to 4096 (F64) / 8192 (F32)
                                     points by Bluestein's algorithm, or up to 2^20 points as a
                                     product of two lengths each within 4096 / 8192 points with
                                     such prime factors or within 2048 / 4096 points by Bluestein

@@ -65,19 +65,25 @@ def test_general_parity(gpu, nx, ny, prec, tol):
 
 
 def test_general_fp32_beyond_fp64_lds(gpu):
-    """FP32 holds twice the points per workgroup: Nx = 6000 (FP64 rejects it)."""
-    nx, ny = 6000, 8
+    """FP32 holds twice the points per workgroup: Nx = 12000 = 2^5 3 5^3 runs
+    one-level in place in FP32, where FP64 takes two levels."""
+    nx, ny = 12000, 8
     x = oracle.fill(nx, ny, seed=11, dist="normal")
     got = _run(x, nx, ny, "f32")[:, : 2 * (ny // 2 + 1)].copy().view(np.complex128)
     assert oracle.rel_l2(got, oracle.r2c_2d(x, ny)) <= 1e-5
 
 
-# lengths beyond one workgroup: two levels N = P*Q (phase A over m1 + P*e with
-# twiddle w_N^{m1 k2}, phase B over m1), rows then split in place
+# lengths beyond one workgroup's pair of buffers: one buffer in place when the
+# radices are at most 7 and the length fits it (6000, 4500 in FP64), else two
+# levels N = P*Q (phase A over m1 + P*e with twiddle w_N^{m1 k2}, phase B over
+# m1), rows then split in place
 SIZES2 = [
-    (6000, 16),       # columns 6000 = 75 * 80
+    (6000, 16),       # columns 6000: in place (FP64 and FP32)
     (16, 12000),      # rows Ny/2 = 6000
-    (3000, 9000),     # direct columns, two-level rows 4500 = 60 * 75
+    (3000, 9000),     # direct columns, rows 4500
+    (18000, 16),      # columns 18000 = 120 * 150: two levels in both precisions
+    (16, 36000),      # rows Ny/2 = 18000
+    (300, 34000),     # rows 17000 = 2^3 5^3 17: two levels
     (6117, 6),        # 3 * 2039: a Bluestein phase inside two levels
     (1 << 17, 4),     # powers of two beyond the power-of-two kernels
     (4, 1 << 17),
@@ -106,8 +112,14 @@ def test_two_level_parity(gpu, nx, ny, prec, tol):
         big = max(big, m)
         return (n <= cap and big <= 61) or 1 << (2 * n - 2).bit_length() <= cap
 
-    def two(n, pow2_max):
-        return not (n & (n - 1) == 0 and n <= pow2_max) and not one_level(n)
+    def two(n, pow2_max):   # (every length here splits)
+        m, f, big = n, 2, 1
+        while f * f <= m:
+            while m % f == 0:
+                big, m = max(big, f), m // f
+            f += 1
+        in_place = n <= 2 * cap and max(big, m) <= 7
+        return not (n & (n - 1) == 0 and n <= pow2_max) and not one_level(n) and not in_place
 
     p = hpxfft_amd.Plan(nx, ny, precision=prec)
     assert ("two levels" in p.describe()) == (two(nx, 1 << 16) or two(ny // 2, 1 << 14))
@@ -208,8 +220,10 @@ def test_two_level_fp32_bluestein_phase(gpu):
     assert oracle.rel_l2(got, oracle.r2c_2d(x, ny)) <= 1e-5
 
 
-def test_two_level_batched(gpu):
-    nx, ny, batch = 6000, 2 * 4500, 2
+@pytest.mark.parametrize("nx,ny", [(6000, 2 * 4500),    # one buffer in place, both dimensions
+                                   (18000, 20), (12, 2 * 18000)])   # two-level columns / rows
+def test_two_level_batched(gpu, nx, ny):
+    batch = 2
     ld = 2 * (ny // 2 + 1)
     xs = [oracle.fill(nx, ny, seed=600 + b, dist="upm1") for b in range(batch)]
     got = _run(np.concatenate(xs), nx, ny, "f64", batch=batch)
@@ -218,7 +232,7 @@ def test_two_level_batched(gpu):
         assert oracle.rel_l2(g, oracle.r2c_2d(x, ny)) <= 1e-12, b
 
 
-@pytest.mark.parametrize("nx,ny", [(6000, 16), (16, 12000), (3000, 9000), (6117, 6)])
+@pytest.mark.parametrize("nx,ny", [(6000, 16), (16, 12000), (3000, 9000), (6117, 6), (18000, 16), (16, 36000)])
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5)])
 def test_two_level_adjoint_and_c2r(gpu, nx, ny, prec, tol):
     import torch

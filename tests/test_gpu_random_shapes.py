@@ -65,7 +65,7 @@ def _one_level(n, cap):
 def _supported(n, cap, pow2_max):
     if n & (n - 1) == 0 and n <= pow2_max:
         return True
-    if _one_level(n, cap):
+    if _one_level(n, cap) or (n <= 2 * cap and _largest_prime(n) <= 7):   # in place, one buffer
         return True
     if n <= 1 << 20 and any(n % p == 0 and n // p >= 2 and _one_level(p, cap) and _one_level(n // p, cap)
                             for p in range(2, min(cap, n // 2) + 1)):
