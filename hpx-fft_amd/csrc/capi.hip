@@ -336,7 +336,8 @@ int gen_allow_lds(int dev)
         GEN_BOTH(gen_col_kernel, float, __half, false), GEN_BOTH(gen_col_kernel, float, bf16_t, false),
         GEN_BOTH(gen_row_c2r_kernel, double, false), GEN_BOTH(gen_row_c2r_kernel, double, true),
         GEN_BOTH(gen_row_c2r_kernel, float, false), GEN_BOTH(gen_row_c2r_kernel, float, true),
-        GEN_BOTH(gen_strided_kernel, double), GEN_BOTH(gen_strided_kernel, float)};
+        GEN_BOTH(gen_strided_kernel, double), GEN_BOTH(gen_strided_kernel, float),
+        GEN_BOTH(gen_rows_split_kernel, double), GEN_BOTH(gen_rows_split_kernel, float)};
 #undef GEN_BOTH
     // the buffers plus up to 1024 words of per-row / per-column maxima
     for (const void *f : fns)
@@ -607,6 +608,8 @@ std::string gen_method(const hpxfft_dev::GenPlan &g)
            radices(g) + (g.ip ? ", in place: decimation in frequency, then in time)" : ", Stockham)");
 }
 
+int rows_split_width(const hpxfft_plan_s *p, hpxfft_dev::GenPlan &gB);
+
 std::string describe(const hpxfft_plan_s *p)
 {
     std::ostringstream os;
@@ -642,7 +645,12 @@ std::string describe(const hpxfft_plan_s *p)
     if (p->g2r.on)
     {
         two_level("row r2c", p->g2r);
-        os << "    then the r2c split in place (one thread per pair k, Ny/2-k)\n";
+        hpxfft_dev::GenPlan gB;
+        if (const int Wh = rows_split_width(p, gB))
+            os << "    forward: phase B with the r2c split fused in, " << Wh << " columns k2 <= Q/2 and their "
+               << "partners Q-k2 per workgroup, " << gen_method(gB) << "\n";
+        else
+            os << "    then the r2c split in place (one thread per pair k, Ny/2-k)\n";
     }
     else if (p->gen_row)
         os << "  row r2c kernel (general length): " << p->ny / 2 << "-point " << gen_method(p->grow) << ", "
@@ -773,6 +781,20 @@ int run_columns_2level(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bo
     return launch_strided(p, b, d.B, d.tabB, d.tabT, d.WB, cdiv(nc, d.WB), (unsigned)Q, (unsigned)p->batch);
 }
 
+// forward two-level rows: phase B with the r2c split fused in, each
+// workgroup taking Wh columns k2 <= Q/2 and their partners Q - k2 (twice the
+// sequences of the plain phase B), in place where the plan allows (gB), with
+// Wh halved until the block fits the LDS; 0 if it never does (then the plain
+// phase B and a split pass run)
+int rows_split_width(const hpxfft_plan_s *p, hpxfft_dev::GenPlan &gB)
+{
+    gB = p->g2r.B;
+    gB.ip = gen_can_ip(gB) ? 1 : 0;
+    int Wh = p->g2r.WB;
+    while (Wh > 1 && gen_lds_bytes(p, gB, (size_t)2 * Wh * gB.n) > (size_t)hpxfft_dev::GEN_LDS_BYTES) Wh /= 2;
+    return gen_lds_bytes(p, gB, (size_t)2 * Wh * gB.n) <= (size_t)hpxfft_dev::GEN_LDS_BYTES ? Wh : 0;
+}
+
 // two-level rows (Ny/2 = L = P*Q).  Forward: the real rows of `real` as
 // complex points -> plan scratch -> the spectrum rows of `spec`, then the
 // r2c split in place.  Backward (c2r, halve for the adjoint): the c2r
@@ -810,6 +832,32 @@ int run_rows_2level(hpxfft_plan_s *p, const void *real_in, void *real_out, const
     a.tw_col = 1;
     if (int rc = launch_strided(p, a, d.A, d.tabA, d.tabT, d.WA, cdiv(P, d.WA), (unsigned)p->nx, (unsigned)p->batch))
         return rc;
+    hpxfft_dev::GenPlan gB;
+    const int Wh = rows_split_width(p, gB);
+    if (!backward && Wh > 0)
+    {
+        hpxfft_dev::GenStride b{};
+        b.in = p->gtmp;
+        b.in_seq = Q;
+        b.in_col = 1;
+        b.in_sub = L;
+        b.in_bat = (long long)p->nx * L;
+        b.out = spec_out;
+        b.out_sub = (long long)p->ld_out;
+        b.out_bat = (long long)p->dist_out;
+        const size_t lds = gen_lds_bytes(p, gB, (size_t)2 * Wh * gB.n);
+        const unsigned nt = gen_threads(lds, gB);
+        const dim3 grid(cdiv((size_t)Q / 2 + 1, (size_t)Wh), (unsigned)p->nx, (unsigned)p->batch);
+        const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
+        if (p->prec == P_F64)
+            GEN_LAUNCH(hpxfft_dev::gen_rows_split_kernel, (double), nt, grid, lds, p->stream, b, gB, tab + d.tabB,
+                       tab + p->gtab_2, Wh, (int)Q);
+        else
+            GEN_LAUNCH(hpxfft_dev::gen_rows_split_kernel, (float), nt, grid, lds, p->stream, b, gB, tab + d.tabB,
+                       tab + p->gtab_2, Wh, (int)Q);
+        HIPCHK(hipGetLastError());
+        return HPXFFT_OK;
+    }
     hpxfft_dev::GenStride b{};
     b.in = p->gtmp;
     b.in_seq = Q;

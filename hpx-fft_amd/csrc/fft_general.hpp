@@ -806,6 +806,71 @@ __global__ void __launch_bounds__(NTMAX) gen_strided_kernel(GenStride a, GenPlan
     }
 }
 
+// Phase B of two-level forward rows with the r2c split fused in (L = P Q,
+// phase A's output in a.in as [row][m1][k2]): output k = Q k1 + k2 pairs
+// with L - k = Q k1' + k2', where k2' = Q - k2 and k1' = P - 1 - k1 for
+// k2 > 0, k2' = 0 and k1' = P - k1 for k2 = 0.  A workgroup takes Wh
+// columns k2 <= Q/2 of one row (blockIdx.x = strip, XCD order; .y = row,
+// .z = grid) and, in slots Wh.., their partners Q - k2, so it holds Z[k] and
+// Z[L-k] for each of its pairs and writes X[k] and X[L-k] (the split as in
+// gen_row_r2c_kernel; X[0], X[L] from Z[0]; X[L/2] = conj Z[L/2]).  A
+// self-paired column (k2 = 0, or Q/2) writes the pairs with 2k <= L.
+// tab: phase B's plan table; tab2: w_{Ny}^m.  Dynamic LDS: gen_nbuf(g) * 2 Wh
+// * g.n complex.
+template <typename T, int NTMAX = GEN_NT>
+__global__ void __launch_bounds__(NTMAX) gen_rows_split_kernel(GenStride a, GenPlan g, const cpx<double> *tab,
+                                                               const cpx<double> *tab2, int Wh, int Q)
+{
+    cpx<T> *buf = gen_lds<T>();
+    const int t = threadIdx.x, P = g.len, ST = g.n, W = 2 * Wh, L = P * Q, H2 = Q / 2 + 1;
+    const int s0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * Wh;
+    const long long sub = blockIdx.y, bat = blockIdx.z;
+    const T *src = reinterpret_cast<const T *>(a.in) + 2 * (bat * a.in_bat + sub * a.in_sub);
+    T *dst = reinterpret_cast<T *>(a.out) + 2 * (bat * a.out_bat + sub * a.out_sub);
+    const float inv_w = 1.0f / (float)W, inv_wh = 1.0f / (float)Wh;
+    for (int i = t; i < P * W; i += gen_nth<NTMAX>())
+    {
+        const int e = gdiv(i, W, inv_w), w = i - e * W;
+        const int kl = s0 + (w < Wh ? w : w - Wh);
+        const int k2 = w < Wh ? kl : (kl == 0 ? 0 : Q - kl);
+        cpx<T> z = {T(0), T(0)};
+        if (kl < H2) z = stor<T>::ld(src + 2 * ((long long)e * a.in_seq + (long long)k2 * a.in_col));
+        buf[w * ST + gen_pos(g, tab, e)] = z;
+    }
+    __syncthreads();
+    const cpx<T> *X = gen_fft<T, NTMAX>(buf, buf + ST * W, W, g, tab);
+    for (int i = t; i < P * Wh; i += gen_nth<NTMAX>())
+    {
+        const int k1 = gdiv(i, Wh, inv_wh), w = i - k1 * Wh;
+        const int kl = s0 + w;
+        if (kl >= H2) continue;
+        const int k = Q * k1 + kl;
+        const bool self = kl == 0 || 2 * kl == Q;
+        if (self && 2 * k > L) continue;
+        const cpx<T> Zk = X[w * ST + k1];
+        if (k == 0)
+        {
+            stor<T>::st(dst, cpx<T>{Zk.x + Zk.y, T(0)});
+            stor<T>::st(dst + 2 * L, cpx<T>{Zk.x - Zk.y, T(0)});
+            continue;
+        }
+        if (2 * k == L)
+        {
+            stor<T>::st(dst + 2 * k, conjc(Zk));
+            continue;
+        }
+        const int k1p = kl == 0 ? P - k1 : P - 1 - k1;
+        const int wp = self ? w : Wh + w;
+        const cpx<T> Bc = conjc(X[wp * ST + k1p]);
+        const cpx<T> Fe = scal(Zk + Bc, T(0.5));
+        const cpx<T> D = scal(Zk - Bc, T(0.5));
+        const cpx<T> Fo = {D.y, -D.x};
+        const cpx<T> tt = cmul(tabv<T>(tab2, k), Fo);
+        stor<T>::st(dst + 2 * k, Fe + tt);
+        stor<T>::st(dst + 2 * (L - k), conjc(Fe - tt));
+    }
+}
+
 // the r2c split of a row's L-point complex transform Z, in place in the
 // spectrum row (positions 0..L):
 //   X[k] = Fe + w_{Ny}^k Fo, X[L-k] = conj(Fe - w_{Ny}^k Fo), X[0], X[L] from Z[0].

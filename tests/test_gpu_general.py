@@ -84,6 +84,9 @@ SIZES2 = [
     (18000, 16),      # columns 18000 = 120 * 150: two levels in both precisions
     (16, 36000),      # rows Ny/2 = 18000
     (300, 34000),     # rows 17000 = 2^3 5^3 17: two levels
+    (6, 2 * 9801),    # rows 99 * 99 (P, Q odd; the r2c split fused into phase B pairs k2 with Q - k2)
+    (4, 2 * 10000),   # rows 100 * 100 (P, Q even) in FP64
+    (5, 2 * 8575),    # rows 49 * 175 in FP64
     (6117, 6),        # 3 * 2039: a Bluestein phase inside two levels
     (1 << 17, 4),     # powers of two beyond the power-of-two kernels
     (4, 1 << 17),
