@@ -798,8 +798,12 @@ __global__ void __launch_bounds__(NTMAX) gen_strided_kernel(GenStride a, GenPlan
         cpx<T> z = X[w * ST + k];
         if (a.tw_n > 0)
         {
-            const long long f = a.tw_col ? (long long)(col0 + w) : sub;
-            z = cmul(z, tabv<T>(twt, (int)((f * k) % a.tw_n)));
+            // f < P and k < Q in both uses (m1 k2 < N <= 2^20), so the
+            // product is already reduced; the modulo only guards other callers
+            const int f = a.tw_col ? col0 + w : (int)sub;
+            int m = f * k;
+            if (m >= a.tw_n) m %= a.tw_n;
+            z = cmul(z, tabv<T>(twt, m));
         }
         if (a.conj_out) z = conjc(z);
         stor<T>::st(dst + 2 * ((long long)k * a.out_seq + (long long)w * a.out_col), z);
