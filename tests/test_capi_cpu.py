@@ -157,6 +157,19 @@ def test_bad_arguments():
     assert L.hpxfft_plan_flag_from_string(None, ctypes.byref(eff)) == _lib.INVALID_ARG
 
 
+def test_header_builds_as_c99(tmp_path):
+    """include/hpxfft_hip.h is plain C: a C99 program includes it, links the
+    library and calls the entry points that need no device."""
+    exe = tmp_path / "abi_c99"
+    lib_dir = os.path.dirname(_lib.LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror",
+                    "-I", os.path.join(REPO, "include"), os.path.join(REPO, "tests", "c", "abi_c99.c"),
+                    "-L", lib_dir, "-lhpxfft_hip", f"-Wl,-rpath,{lib_dir}", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+    assert "hpxfft-hip" in r.stdout
+
+
 def test_no_device_is_reported_cleanly():
     st = plan_status(64, 64)
     if st == _lib.OK:
