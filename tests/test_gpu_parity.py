@@ -100,7 +100,9 @@ def test_pointers_off_a_complex_element(gpu, prec, dt):
 
 @pytest.mark.parametrize("nx,ny,batch", [(128, 256, 3), (1024, 64, 2), (4096, 128, 3), (16, 4096, 5),
                                          # general one-level, Bluestein and two-level lengths
-                                         (30, 42, 3), (67, 134, 2), (6000, 20, 2), (12, 2 * 6000, 2)])
+                                         (30, 42, 3), (67, 134, 2), (6000, 20, 2), (12, 2 * 6000, 2),
+                                         # two levels in both precisions (rows: phase B with the split fused in)
+                                         (18000, 20, 2), (12, 2 * 18000, 2)])
 @pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
 def test_parity_batched_strided(gpu, nx, ny, batch, prec, tol):
     """Several grids in one plan, out of place, with gaps between grids and
