@@ -43,7 +43,11 @@ def to16(x_pad, ny, dt, scale_rows):
                                        (32768, 256, 1.0),
                                        # lengths that are not powers of two (general kernels)
                                        (8, 14, 1.0), (100, 134, 1e6), (67, 256, 1.0), (3000, 64, 1e6),
-                                       (64, 3000, 1.0), (1000, 1000, 1e6), (6000, 16, 1.0), (4096, 30, 1e6)])
+                                       (64, 3000, 1.0), (1000, 1000, 1e6), (6000, 16, 1.0), (4096, 30, 1e6),
+                                       # one LDS buffer, in place: 12000 = 2^5 3 5^3, and 8191 (prime)
+                                       # through a 16384-point Bluestein FFT; 4093 through 8192 points
+                                       (12000, 16, 1e6), (8, 2 * 12000, 1.0), (8191, 8, 1.0),
+                                       (16, 2 * 4093, 1e6)])
 def test_16bit_storage(gpu, dt_name, bound, nx, ny, amp):
     import torch
     import hpxfft_amd
