@@ -12,7 +12,8 @@ value = algorithmic bytes / time, 2-pass model (transposes not credited):
 Prints ONE JSON line.  Flags: --steps K --warmup W --gpus N (N must be 1)
 --no-cpu (skip the CPU baseline leg) --no-f32 --no-extra (skip the other
 legs) --nx/--ny (other sizes) --copy-probe BYTES (only a device copy, for
-calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE).
+calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE) --general-only (only the
+general-length legs, one JSON line).
 
 roofline: the dominant kernel, the pass with the longest event-timed
 duration on the plan's stream (a four-step column pass is one kernel in two
@@ -301,6 +302,15 @@ def bench_general(nx: int = 3000, ny: int = 3000, reps: int = 10) -> dict:
             "kernels": desc}
 
 
+def general_legs() -> dict:
+    """The general-length legs: mixed radix, Bluestein, long lengths in one
+    LDS buffer, and two levels."""
+    return {"general_length": bench_general(),
+            "general_length_bluestein": bench_general(2039, 2 * 2039),
+            "general_length_long": bench_general(6000, 12000),
+            "general_length_two_level": bench_general(9000, 18000)}
+
+
 def cpu_baseline(nx: int, ny: int, budget_s: float = 20.0):
     """The oracle's restatement of shared::loop::fft_2d_r2c_par on host cores."""
     import oracle
@@ -335,6 +345,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-f32", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the batched / adjoint / 2^15 mixed-precision legs")
+    ap.add_argument("--general-only", action="store_true",
+                    help="only the lengths that are not powers of two (FP64, plans at measure): one JSON line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (reads BYTES, writes BYTES) three times")
     args = ap.parse_args()
@@ -349,6 +361,9 @@ def main() -> int:
         return 2
     if args.copy_probe:
         return copy_probe(args.copy_probe)
+    if args.general_only:
+        print(json.dumps(general_legs()), flush=True)
+        return 0
     nx, ny = args.nx, args.ny
     r64 = time_gpu(nx, ny, "f64", args.steps, args.warmup, seed=2)
     progress(f"f64 {nx}x{ny}: {r64['ms_per_step']:.4f} ms/step")
@@ -438,11 +453,7 @@ def main() -> int:
         result["mixed_2e15"] = bench_mixed()
         progress("2^15 mixed precision done")
         torch.cuda.empty_cache()
-        result["general_length"] = bench_general()
-        progress("general lengths: smooth done")
-        result["general_length_bluestein"] = bench_general(2039, 2 * 2039)
-        result["general_length_long"] = bench_general(6000, 12000)
-        result["general_length_two_level"] = bench_general(9000, 18000)
+        result.update(general_legs())
         progress("general length done")
         torch.cuda.empty_cache()
     if not args.no_cpu:
