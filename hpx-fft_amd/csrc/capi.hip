@@ -312,7 +312,7 @@ unsigned gen_threads(size_t lds, const hpxfft_dev::GenPlan &g)
     // over r inputs: twice the threads hide that loop's latency (measured:
     // rows of 516 = 4*3*43 points, 46 -> 34 us at 512 threads)
     bool generic = false;
-    for (int i = 0; i < g.nstages; ++i) generic = generic || g.radix[i] > 7;
+    for (int i = 0; i < g.nstages; ++i) generic = generic || g.radix[i] > 8;
     if (per_cu >= 3) return generic ? 512u : 256u;
     return per_cu == 2 ? 512u : 1024u;
 }
@@ -346,8 +346,13 @@ int gen_allow_lds(int dev)
     return HPXFFT_OK;
 }
 
-// Stockham radices for a length that is not a power of two: 4s, then a 2,
-// then odd primes ascending; false if a prime factor exceeds GEN_MAX_RADIX
+// Stockham radices for a length that is not a power of two: for its 2^e
+// part 8s (then a 4 or a 2) where they take fewer stages than 4s (then a 2),
+// else 4s; then odd primes ascending; false if a prime factor exceeds
+// GEN_MAX_RADIX.  A radix-8 stage is heavier than a radix-4 one, so it pays
+// only by removing a stage (measured: 4096-point Bluestein FFTs 6 -> 4
+// stages, 2039 x 4078 FP64 0.315 -> 0.296 ms; 6000 = 8*2*3*5^3 in as many
+// stages as 4*4*3*5^3 was 6% slower)
 bool gen_factor(size_t n, hpxfft_dev::GenPlan &g)
 {
     g = hpxfft_dev::GenPlan{};
@@ -358,6 +363,10 @@ bool gen_factor(size_t n, hpxfft_dev::GenPlan &g)
         g.radix[g.nstages++] = r;
         return true;
     };
+    int e2 = 0;
+    for (size_t r = m; r % 2 == 0; r /= 2) ++e2;
+    if ((e2 + 2) / 3 < (e2 + 1) / 2)
+        while (m % 8 == 0) { if (!push(8)) return false; m /= 8; }
     while (m % 4 == 0) { if (!push(4)) return false; m /= 4; }
     if (m % 2 == 0) { if (!push(2)) return false; m /= 2; }
     for (size_t f = 3; m > 1; f += 2)
@@ -389,7 +398,7 @@ bool gen_setup(size_t n, size_t cap, hpxfft_dev::GenPlan &g, size_t ip_cap = 0)
     if (n > cap && n <= ip_cap && gen_factor(n, g))
     {
         long_ok = true;
-        for (int i = 0; i < g.nstages; ++i) long_ok = long_ok && g.radix[i] <= 7;
+        for (int i = 0; i < g.nstages; ++i) long_ok = long_ok && g.radix[i] <= 8;
     }
     if ((n <= cap || long_ok) && gen_factor(n, g))
     {
@@ -402,7 +411,7 @@ bool gen_setup(size_t n, size_t cap, hpxfft_dev::GenPlan &g, size_t ip_cap = 0)
         // workgroup 117 -> 79 us, rows of 1500 points 70 -> 58 us; rows of
         // 4 x 500 points 16 -> 17 us, the one case measured slower
         bool small = true;
-        for (int i = 0; i < g.nstages; ++i) small = small && g.radix[i] <= 7;
+        for (int i = 0; i < g.nstages; ++i) small = small && g.radix[i] <= 8;
         g.off_r = small ? g.n : -1;
         g.off_bp = -1;
         g.ip = small ? 1 : 0;

@@ -12,10 +12,10 @@
 //   stage with sub-length Ns, butterfly j < n/r, k = j mod Ns:
 //     x_q = A[j + q n/r] w_{r Ns}^{q k},  y = DFT_r(x),
 //     B[(j / Ns) r Ns + k + q Ns] = y_q
-// Radices 2, 3, 4, 5 and 7 run as register butterflies (one LDS pass per
+// Radices 2, 3, 4, 5, 7 and 8 run as register butterflies (one LDS pass per
 // stage); larger primes (up to 61) by a generic stage of two passes.
 //
-// Direct plans whose radices are all at most 7 can run in place instead, in
+// Direct plans whose radices are all at most 8 can run in place instead, in
 // one buffer (half the LDS, so twice the workgroups per CU): the loads scatter
 // element e to position pos(e), the mixed-radix digit reversal of e, and
 // decimation-in-time stages r_1, ..., r_S then leave X[k] at position k:
@@ -107,6 +107,23 @@ template <typename T, int R> __device__ __forceinline__ void dft_small(cpx<T> *x
         x[1] = d0 + mul_mi(d1);
         x[3] = d0 - mul_mi(d1);
     }
+    else if constexpr (R == 8)
+    {
+        // two 4-point DFTs of the even and odd inputs, X[k] = E[k] + w_8^k O[k],
+        // X[k+4] = E[k] - w_8^k O[k] (w_8 = w[1], w_8^2 = -i, w_8^3 = w[3])
+        cpx<T> e[4] = {x[0], x[2], x[4], x[6]}, o[4] = {x[1], x[3], x[5], x[7]};
+        dft_small<T, 4>(e, w);
+        dft_small<T, 4>(o, w);
+        o[1] = cmul(o[1], w[1]);
+        o[2] = mul_mi(o[2]);
+        o[3] = cmul(o[3], w[3]);
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+        {
+            x[k] = e[k] + o[k];
+            x[k + 4] = e[k] - o[k];
+        }
+    }
     else
     {
         cpx<T> y[R];
@@ -127,7 +144,7 @@ template <typename T, int R> __device__ __forceinline__ void dft_small(cpx<T> *x
     }
 }
 
-// one Stockham stage of radix R (2, 3, 4, 5, 7) over nseq sequences, A -> B
+// one Stockham stage of radix R (2, 3, 4, 5, 7, 8) over nseq sequences, A -> B
 template <typename T, int R, int NTMAX>
 __device__ void gen_stage_reg(const cpx<T> *A, cpx<T> *B, int nseq, int n, int Ns, const cpx<double> *__restrict__ tab)
 {
@@ -185,7 +202,7 @@ __device__ void gen_stage_reg(const cpx<T> *A, cpx<T> *B, int nseq, int n, int N
     __syncthreads();
 }
 
-// one in-place decimation-in-time stage of radix R (2, 3, 4, 5, 7) over nseq
+// one in-place decimation-in-time stage of radix R (2, 3, 4, 5, 7, 8) over nseq
 // sequences: butterfly (c, j) reads and writes the R positions
 // c R Ls + j + q Ls, so the butterflies of a stage touch disjoint points
 template <typename T, int R, int NTMAX>
@@ -292,6 +309,7 @@ __device__ cpx<T> *gen_stages(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, 
         case 4: gen_stage_reg<T, 4, NTMAX>(A, B, nseq, g.n, Ns, tab); break;
         case 5: gen_stage_reg<T, 5, NTMAX>(A, B, nseq, g.n, Ns, tab); break;
         case 7: gen_stage_reg<T, 7, NTMAX>(A, B, nseq, g.n, Ns, tab); break;
+        case 8: gen_stage_reg<T, 8, NTMAX>(A, B, nseq, g.n, Ns, tab); break;
         default: gen_stage_any<T, NTMAX>(A, B, nseq, g.n, r, Ns, tab); break;
         }
         cpx<T> *tmp = A;
@@ -360,7 +378,7 @@ __device__ void gen_stage_dif(cpx<T> *A, int nseq, int n, int Lb, const cpx<doub
 }
 
 // all decimation-in-frequency stages (r_S first) of a plan whose radices are
-// at most 7: natural order in A to X[k] at pos(k)
+// at most 8: natural order in A to X[k] at pos(k)
 template <typename T, int NTMAX>
 __device__ void gen_stages_dif(cpx<T> *A, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
 {
@@ -374,13 +392,14 @@ __device__ void gen_stages_dif(cpx<T> *A, int nseq, const GenPlan &g, const cpx<
         case 3: gen_stage_dif<T, 3, NTMAX>(A, nseq, g.n, Lb, tab); break;
         case 4: gen_stage_dif<T, 4, NTMAX>(A, nseq, g.n, Lb, tab); break;
         case 5: gen_stage_dif<T, 5, NTMAX>(A, nseq, g.n, Lb, tab); break;
-        default: gen_stage_dif<T, 7, NTMAX>(A, nseq, g.n, Lb, tab); break;   // the host admits only 2..7 here
+        case 8: gen_stage_dif<T, 8, NTMAX>(A, nseq, g.n, Lb, tab); break;
+        default: gen_stage_dif<T, 7, NTMAX>(A, nseq, g.n, Lb, tab); break;   // the host admits only 2..8 here
         }
         Lb /= r;
     }
 }
 
-// all stages of an in-place plan (radices <= 7) on the digit-reversed
+// all stages of an in-place plan (radices <= 8) on the digit-reversed
 // sequences in A; the result is in A in natural order
 template <typename T, int NTMAX>
 __device__ void gen_stages_ip(cpx<T> *A, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
@@ -395,7 +414,8 @@ __device__ void gen_stages_ip(cpx<T> *A, int nseq, const GenPlan &g, const cpx<d
         case 3: gen_stage_ip<T, 3, NTMAX>(A, nseq, g.n, Ls, tab); break;
         case 4: gen_stage_ip<T, 4, NTMAX>(A, nseq, g.n, Ls, tab); break;
         case 5: gen_stage_ip<T, 5, NTMAX>(A, nseq, g.n, Ls, tab); break;
-        default: gen_stage_ip<T, 7, NTMAX>(A, nseq, g.n, Ls, tab); break;   // the host admits only 2..7 here
+        case 8: gen_stage_ip<T, 8, NTMAX>(A, nseq, g.n, Ls, tab); break;
+        default: gen_stage_ip<T, 7, NTMAX>(A, nseq, g.n, Ls, tab); break;   // the host admits only 2..8 here
         }
         Ls *= r;
     }
