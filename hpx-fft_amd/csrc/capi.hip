@@ -1901,11 +1901,10 @@ int plan_batched(hpxfft_plan_s *p)
             p->bblocks[ph][c] = batched_resident_blocks(ph, c);
         }
     }
-    double pts = 0.0, fl = 0.0;
+    double fl = 0.0;
     for (size_t i = 0; i < n; ++i)
     {
         const double x = (double)p->bnx[i], y = (double)p->bny[i];
-        pts += x * y;
         fl += x * 2.5 * y * std::log2(y) + (y / 2 + 1) * 5.0 * x * std::log2(x);
     }
     p->plan_flops = fl;

@@ -49,7 +49,7 @@ template <typename T, typename S> struct cfg
     // four-step tiles: TW adjacent columns x L rows (row segments of 512 B for
     // FP64 and FP32 storage, 256 B for 16-bit storage)
     static constexpr int TW = sizeof(S) == 8 ? 32 : 64;
-    static constexpr int tile_E(int lg) { return 16; }
+    static constexpr int tile_E(int) { return 16; }
     static constexpr int tile_NT(int lg) { return (TW << lg) / 16; }
     static constexpr bool tile_SPLIT(int lg) { return (long)(TW << lg) * 2 * (long)sizeof(T) > lds_budget; }
 };
