@@ -75,7 +75,8 @@ class Plan:
         check(lib().hpxfft_exec_c2r_2d(self._h, ctypes.c_void_p(d_spec), ctypes.c_void_p(d_real)))
 
     def exec_host(self, values: np.ndarray) -> None:
-        assert values.dtype == np.float64 and values.flags.c_contiguous
+        if values.dtype != np.float64 or not values.flags.c_contiguous:
+            raise ValueError("exec_host takes a C-contiguous float64 array (the padded Nx x 2(Ny/2+1) buffer)")
         check(lib().hpxfft_exec_r2c_2d_host(self._h, values.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
 
     def measurement(self, key: str) -> float:
@@ -117,7 +118,8 @@ class BatchedPlan:
     """
 
     def __init__(self, nx, ny, *, effort: int | str = "estimate", stream=None):
-        assert len(nx) == len(ny) and len(nx) > 0
+        if len(nx) != len(ny) or len(nx) == 0:
+            raise ValueError("need equal, non-empty lists of Nx and Ny")
         self.nx, self.ny = [int(v) for v in nx], [int(v) for v in ny]
         n = len(self.nx)
         eff = plan_flag_to_effort(effort) if isinstance(effort, str) else int(effort)
@@ -129,6 +131,8 @@ class BatchedPlan:
         self._ptrs = (ctypes.c_void_p * n)()
 
     def exec(self, d_grids) -> None:
+        if len(d_grids) != len(self.nx):
+            raise ValueError(f"need {len(self.nx)} device pointers, one per grid")
         for i, p in enumerate(d_grids):
             self._ptrs[i] = p
         check(lib().hpxfft_exec_r2c_2d_batched(self._h, self._ptrs))
