@@ -1,10 +1,11 @@
 // fft_general.hpp -- 2-D r2c passes for lengths that are not powers of two
-// (FP64 / FP32).  The reference accepts any even Ny and any Nx through FFTW
-// (core/src/util/adapter_fftw.cpp:9,28-29); these kernels cover the lengths
-// whose prime factors are at most GEN_MAX_RADIX and whose data fit one
-// workgroup's LDS buffer, and any other length whose Bluestein FFT (a power
-// of two >= 2 len - 1) fits it, with the power-of-two kernels kept for
-// power-of-two dimensions.
+// (FP64 / FP32, and 16-bit storage within one workgroup).  The reference
+// accepts any even Ny and any Nx through FFTW (core/src/util/adapter_fftw.cpp:
+// 9,28-29); these kernels cover the lengths whose prime factors are at most
+// GEN_MAX_RADIX and whose data fit one workgroup's LDS, any other length
+// whose Bluestein FFT (a power of two >= 2 len - 1) fits it, and longer
+// lengths as two levels N = P Q (gen_strided_kernel, gen_rows_split_kernel),
+// with the power-of-two kernels kept for power-of-two dimensions.
 //
 // One workgroup transforms NSEQ sequences of length n held in LDS (sequence
 // s at [s*n, (s+1)*n), two ping-pong buffers in dynamic LDS sized to the
@@ -30,7 +31,7 @@
 // stages bring the inverse transform back to natural order.
 // Twiddles and the roots w_r^e = w_n^{e n/r} come from the plan's exact FP64
 // table (rounded once for FP32); a register butterfly loads w^k and forms
-// w^{qk} (q < r <= 7) as FP64 powers of it.
+// w^{qk} (q < r <= 8) as FP64 powers of it.
 #pragma once
 
 #include "fft_kernels.hpp"
