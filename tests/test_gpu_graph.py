@@ -14,7 +14,10 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("nx,ny,prec", [(2048, 2048, "f64"), (512, 4096, "f32"), (300, 3000, "f64"),
-                                        (6000, 16, "f64")])
+                                        (6000, 16, "f64"),       # one LDS buffer, in place
+                                        (67, 134, "f64"),        # Bluestein, in place
+                                        (18000, 16, "f64"),      # two-level columns
+                                        (8, 2 * 18000, "f64")])  # two-level rows, split fused into phase B
 def test_forward_graph_replays(gpu, nx, ny, prec):
     import torch
     import hpxfft_amd
