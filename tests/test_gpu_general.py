@@ -214,6 +214,21 @@ def test_fp32_in_place_only_bluestein(gpu, nx, ny):
         assert err <= 1e-5, err
 
 
+@pytest.mark.parametrize("nx,ny,text", [
+    (8, 2 * 2039, "4096-point FFT in LDS, radices 8x8x8x8"),   # 2^12: 4 radix-8 stages, not 6 radix-4
+    (6000, 8, "6000-point mixed-radix in-place decimation in time in LDS, radices 4x4x3x5x5x5"),  # 2^4: 4s
+    (3000, 8, "radices 8x3x5x5x5"),                             # 2^3: one radix-8 stage
+    (1500, 8, "radices 4x3x5x5x5")])                            # 2^2: one radix-4 stage
+def test_radix_choice_in_plan_text(gpu, nx, ny, text):
+    """The power-of-two part of a general length takes radix-8 stages only
+    where they remove a stage (the plan text lists the radices)."""
+    import hpxfft_amd
+    p = hpxfft_amd.Plan(nx, ny, precision="f64")
+    desc = p.describe()
+    p.close()
+    assert text in desc, desc
+
+
 def test_two_level_fp32_bluestein_phase(gpu):
     """FP32 Bluestein FFTs of up to 8192 points: 8186 = 2 * 4093 (prime) as a
     2-point phase and a 4093-point Bluestein phase."""
