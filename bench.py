@@ -278,6 +278,42 @@ def _split_line(p) -> list:
     return [ln.strip() for ln in p.describe().splitlines() if "four-step" in ln or "autotuned" in ln]
 
 
+def bench_c1(nx: int = 1 << 8, ny: int = 1 << 14, seed: int = 1, reps: int = 50, per_graph: int = 20) -> dict:
+    """The reference harness's shape family read as exponents
+    (shared_loop_2d.cpp:142-143, README sizes 8 and 14): 2^8 x 2^14 FP64,
+    U[0,1), plan at the harness default "estimate".  34 MB in and out, so
+    launch latency is a visible share: timed eagerly (one exec per step, back
+    to back) and as a captured HIP graph of `per_graph` execs replayed."""
+    import torch
+    import hpxfft_amd
+
+    ld = 2 * (ny // 2 + 1)
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    x = torch.empty((nx, ld), dtype=torch.float64, device="cuda")
+    x[:, :ny].uniform_(0.0, 1.0, generator=g)
+    x[:, ny:].zero_()
+    out = torch.empty((nx, ny // 2 + 1), dtype=torch.complex128, device="cuda")
+    s = torch.cuda.Stream()
+    p = hpxfft_amd.Plan(nx, ny, ld_in=ld, precision="f64", effort="estimate", stream=s)
+    with torch.cuda.stream(s):
+        eager = _events_ms(lambda: p.exec(x.data_ptr(), out.data_ptr()), reps, warmup=5)
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph, stream=s):
+            for _ in range(per_graph):
+                p.exec(x.data_ptr(), out.data_ptr())
+        graphed = _events_ms(graph.replay, max(3, reps // per_graph), warmup=2) / per_graph
+        row_s, col_s = p.measurement("first_fftw"), p.measurement("second_fftw")
+    s.synchronize()
+    desc = [ln.strip() for ln in p.describe().splitlines() if "kernel" in ln]
+    p.close()
+    B = alg_bytes(nx, ny, 8)["total"]
+    return {"config": f"{nx}x{ny} FP64 U[0,1) seed {seed}, plan estimate", "alg_bytes": B,
+            "ms_eager": round(eager, 5), "ms_graph": round(graphed, 5),
+            "GBps_eager": round(B / (eager * 1e-3) / 1e9, 1), "GBps_graph": round(B / (graphed * 1e-3) / 1e9, 1),
+            "hbm_frac_graph": round(B / (graphed * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "phases_s_last_eager_exec": {"row_r2c": row_s, "column_c2c": col_s}, "kernels": desc}
+
+
 def bench_general(nx: int = 3000, ny: int = 3000, reps: int = 10) -> dict:
     """Lengths that are not powers of two, FP64, on the general kernels
     (2-pass byte model): by default 3000 x 3000 (radices 4, 2, 3, 5); with
@@ -315,7 +351,7 @@ def cpu_baseline(nx: int, ny: int, budget_s: float = 20.0):
     """The oracle's restatement of shared::loop::fft_2d_r2c_par on host cores."""
     import oracle
 
-    threads = min(16, len(os.sched_getaffinity(0)))
+    threads = len(os.sched_getaffinity(0))   # every core this process may run on
     x = oracle.fill(nx, ny, seed=2, dist="upm1")
     loop = oracle.Loop(nx, ny)
     t0 = time.perf_counter()
@@ -331,7 +367,7 @@ def cpu_baseline(nx: int, ny: int, budget_s: float = 20.0):
     B = alg_bytes(nx, ny, 8)["total"]
     return {"value": B / t / 1e9, "unit": "GB/s", "cores": threads, "kind": "port",
             "sample": f"full {nx}x{ny} FP64 grid, oracle restatement of shared::loop::fft_2d_r2c_par "
-                      f"(OpenMP, radix-2 FFT), median of {reps} after 1 warm-up",
+                      f"(OpenMP over all {threads} affinity cores, radix-2 FFT), median of {reps} after 1 warm-up",
             "seconds": t, "phases": dict(loop.measurements)}
 
 
@@ -444,6 +480,8 @@ def main() -> int:
                          "phases_s": {"row_r2c": r32["row_s"], "column_c2c": r32["col_s"]}}
     if not args.no_extra:
         torch.cuda.empty_cache()
+        result["readme_shape_2e8x2e14"] = bench_c1()
+        progress("2^8 x 2^14 done")
         result["batched_uneven"] = bench_batched()
         progress("batched uneven grids done")
         torch.cuda.empty_cache()

@@ -16,7 +16,9 @@
 // <= 61 directly and others by Bluestein's algorithm; longer lengths up to
 // 2^20 as a product of two such); otherwise the class throws
 // std::invalid_argument and the program exits with status 2.
-#include "hpxfft/shared/loop.hpp"
+#include "hpxfft/shared/loop.hpp"          // for hpxfft::shared::loop, hpxfft::shared::vector_2d
+#include "hpxfft/util/create_dir.hpp"      // for hpxfft::util::create_parent_dir
+#include "hpxfft/util/print_vector.hpp"    // for hpxfft::util::print_vector_2d
 
 #include <cctype>
 #include <chrono>
@@ -28,12 +30,6 @@
 #include <map>
 #include <stdexcept>
 #include <string>
-
-static void create_parent_dir(const std::string &path)
-{
-    const std::filesystem::path p(path);
-    if (p.has_parent_path()) std::filesystem::create_directories(p.parent_path());
-}
 
 // program_options' bool words; throws std::invalid_argument otherwise.
 static bool parse_flag(const std::string &key, std::string v)
@@ -60,22 +56,6 @@ static std::size_t parse_size(const std::string &key, const std::string &v)
     if (used == 0 || used != v.size())
         throw std::invalid_argument("the argument ('" + v + "') for option '--" + key + "' is invalid");
     return (std::size_t)x;
-}
-
-static void print_vector_2d(const hpxfft::shared::vector_2d &v)
-{
-    std::size_t counter = 0;
-    for (std::size_t i = 0; i < v.n_row(); ++i)
-    {
-        for (std::size_t j = 0; j < v.n_col(); ++j)
-        {
-            if (counter % 2 == 0) std::cout << "(" << v(i, j) << " ";
-            else std::cout << v(i, j) << ") ";
-            ++counter;
-        }
-        std::cout << "\n";
-    }
-    std::cout << std::endl;
 }
 
 int main(int argc, char *argv[])
@@ -144,7 +124,7 @@ int main(int argc, char *argv[])
         std::cerr << "error: " << e.what() << std::endl;
         return 1;
     }
-    if (print_result) print_vector_2d(values_vec);
+    if (print_result) hpxfft::util::print_vector_2d(values_vec);
 
     auto m = [&](const char *k) { return fft_computer.get_measurement(k); };
     std::cout << "\nLocality 0 - shared - " << run_flag << "\n"
@@ -169,14 +149,14 @@ int main(int argc, char *argv[])
                std::to_string(m("plan")) + ";" + std::to_string(m("plan_flops")) + ";\n";
     };
     const std::string runtime_file_path = "runtimes/runtimes_hpx_shared_loop.txt";
-    create_parent_dir(runtime_file_path);
+    hpxfft::util::create_parent_dir(runtime_file_path);
     {
         std::ofstream runtime_file(runtime_file_path, std::ios_base::app);
         if (print_header) runtime_file << header;
         runtime_file << line();
     }
     const std::string plan_file_path = "plans/plan_hpx_shared_loop.txt";
-    create_parent_dir(plan_file_path);
+    hpxfft::util::create_parent_dir(plan_file_path);
     {
         std::ofstream plan_info_file(plan_file_path, std::ios_base::app);
         plan_info_file << header << line();

@@ -718,6 +718,19 @@ std::string describe(const hpxfft_plan_s *p)
     return os.str();
 }
 
+// a HIP event destroyed when it leaves scope
+struct scoped_event
+{
+    hipEvent_t ev = nullptr;
+    scoped_event() = default;
+    scoped_event(const scoped_event &) = delete;
+    scoped_event &operator=(const scoped_event &) = delete;
+    ~scoped_event()
+    {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+};
+
 double elapsed(hipEvent_t a, hipEvent_t b)
 {
     float ms = 0.0f;
@@ -1555,34 +1568,35 @@ int hpxfft_exec_r2c_2d_scaled(hpxfft_plan plan, const void *d_in, void *d_out, c
 
 int hpxfft_exec_r2c_2d_host(hpxfft_plan plan, double *inout)
 {
-    if (!plan || plan->batched) return fail(HPXFFT_INVALID_ARG, "invalid plan");
-    if (!inout) return fail(HPXFFT_INVALID_ARG, "null host pointer");
-    if (plan->prec != P_F64) return fail(HPXFFT_INVALID_ARG, "host exec is for F64 plans");
-    if (plan->ld_in != 2 * plan->ld_out || plan->batch != 1)
-        return fail(HPXFFT_INVALID_ARG, "host exec needs the in-place padded layout (ld_in == 2*(Ny/2+1))");
-    device_guard g(plan->device);
-    const size_t bytes = plan->nx * plan->ld_in * sizeof(double);
-    if (!plan->host_buf || plan->host_bytes < bytes)
-    {
-        (void)hipFree(plan->host_buf);
-        plan->host_buf = nullptr;
-        HIPCHK(hipMalloc(&plan->host_buf, bytes));
-        plan->host_bytes = bytes;
-    }
-    hipEvent_t e0, e1;
-    HIPCHK(hipEventCreate(&e0));
-    HIPCHK(hipEventCreate(&e1));
-    HIPCHK(hipEventRecord(e0, plan->stream));
-    HIPCHK(hipMemcpyAsync(plan->host_buf, inout, bytes, hipMemcpyHostToDevice, plan->stream));
-    if (int rc = exec_forward(plan, plan->host_buf, plan->host_buf, nullptr, nullptr)) return rc;
-    HIPCHK(hipMemcpyAsync(inout, plan->host_buf, bytes, hipMemcpyDeviceToHost, plan->stream));
-    HIPCHK(hipEventRecord(e1, plan->stream));
-    HIPCHK(hipEventSynchronize(e1));
-    plan->h2d = elapsed(e0, plan->ev[0]);
-    plan->d2h = elapsed(plan->ev[2], e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    return HPXFFT_OK;
+    return guarded([&]() -> int {
+        if (!plan || plan->batched) return fail(HPXFFT_INVALID_ARG, "invalid plan");
+        if (!inout) return fail(HPXFFT_INVALID_ARG, "null host pointer");
+        if (plan->prec != P_F64) return fail(HPXFFT_INVALID_ARG, "host exec is for F64 plans");
+        if (plan->ld_in != 2 * plan->ld_out || plan->batch != 1)
+            return fail(HPXFFT_INVALID_ARG, "host exec needs the in-place padded layout (ld_in == 2*(Ny/2+1))");
+        device_guard g(plan->device);
+        const size_t bytes = plan->nx * plan->ld_in * sizeof(double);
+        if (!plan->host_buf || plan->host_bytes < bytes)
+        {
+            (void)hipFree(plan->host_buf);
+            plan->host_buf = nullptr;
+            plan->host_bytes = 0;
+            HIPCHK(hipMalloc(&plan->host_buf, bytes));
+            plan->host_bytes = bytes;
+        }
+        scoped_event e0, e1;   // destroyed on every return path
+        HIPCHK(hipEventCreate(&e0.ev));
+        HIPCHK(hipEventCreate(&e1.ev));
+        HIPCHK(hipEventRecord(e0.ev, plan->stream));
+        HIPCHK(hipMemcpyAsync(plan->host_buf, inout, bytes, hipMemcpyHostToDevice, plan->stream));
+        if (int rc = exec_forward(plan, plan->host_buf, plan->host_buf, nullptr, nullptr)) return rc;
+        HIPCHK(hipMemcpyAsync(inout, plan->host_buf, bytes, hipMemcpyDeviceToHost, plan->stream));
+        HIPCHK(hipEventRecord(e1.ev, plan->stream));
+        HIPCHK(hipEventSynchronize(e1.ev));
+        plan->h2d = elapsed(e0.ev, plan->ev[0]);
+        plan->d2h = elapsed(plan->ev[2], e1.ev);
+        return HPXFFT_OK;
+    });
 }
 
 }   // extern "C"
@@ -1965,6 +1979,22 @@ int hpxfft_plan_batched_set_config(hpxfft_plan plan, int row_cfg, int col_cfg)
         return fail(HPXFFT_INVALID_ARG, "configuration out of range");
     plan->bcfg[0] = row_cfg;
     plan->bcfg[1] = col_cfg;
+    return HPXFFT_OK;
+}
+
+int hpxfft_plan_set_column_split(hpxfft_plan plan, int log2_p)
+{
+    if (!plan || plan->batched || !plan->four_step)
+        return fail(HPXFFT_INVALID_ARG, "the plan's column pass is not a four-step");
+    int lo, hi;
+    split_range(plan->lgx, lo, hi);
+    if (log2_p < lo || log2_p > hi)
+        return fail(HPXFFT_INVALID_ARG, "column split out of range: log2 P in [" + std::to_string(lo) + ", " +
+                                            std::to_string(hi) + "] for Nx = " + std::to_string(plan->nx));
+    // the scratch holds TW-column strips of Nx rows whatever the split, and the
+    // 16-bit column maxima were sized for the largest P of the range
+    plan->lgP = log2_p;
+    plan->lgQ = plan->lgx - log2_p;
     return HPXFFT_OK;
 }
 

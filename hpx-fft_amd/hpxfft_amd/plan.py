@@ -79,6 +79,11 @@ class Plan:
             raise ValueError("exec_host takes a C-contiguous float64 array (the padded Nx x 2(Ny/2+1) buffer)")
         check(lib().hpxfft_exec_r2c_2d_host(self._h, values.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
 
+    def set_column_split(self, log2_p: int) -> None:
+        """Run the four-step column pass with Nx = P*Q, P = 2^log2_p (tests
+        use it to check every split the measure effort may keep)."""
+        check(lib().hpxfft_plan_set_column_split(self._h, int(log2_p)))
+
     def measurement(self, key: str) -> float:
         v = ctypes.c_double(0.0)
         check(lib().hpxfft_get_measurement(self._h, key.encode(), ctypes.byref(v)))

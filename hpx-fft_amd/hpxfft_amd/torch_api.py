@@ -132,8 +132,21 @@ def rfft2_scaled(x16: torch.Tensor, in_row_exp: torch.Tensor | None = None):
     """
     if x16.dtype not in (torch.float16, torch.bfloat16):
         raise TypeError("rfft2_scaled takes float16 or bfloat16")
+    if not x16.is_cuda:
+        raise ValueError("hpxfft_amd operates on GPU tensors (no CPU path)")
+    if x16.dim() != 2:
+        raise ValueError("rfft2_scaled takes one (Nx, 2*(Ny/2+1)) grid")
     nx, ld = x16.shape
     ny = ld - 2
+    if in_row_exp is not None:
+        # the kernels read Nx int32 exponents from this pointer on x16's device
+        if not in_row_exp.is_cuda or in_row_exp.device != x16.device:
+            raise ValueError("in_row_exp must be a tensor on the device of x16")
+        if in_row_exp.dtype != torch.int32:
+            raise TypeError(f"in_row_exp must be int32 (got {in_row_exp.dtype})")
+        if in_row_exp.numel() != nx:
+            raise ValueError(f"in_row_exp needs Nx = {nx} entries (got {in_row_exp.numel()})")
+        in_row_exp = in_row_exp.contiguous()
     buf = x16.contiguous().clone()
     col_exp = torch.empty(ny // 2 + 1, dtype=torch.int32, device=x16.device)
     prec = "f16" if x16.dtype == torch.float16 else "bf16"

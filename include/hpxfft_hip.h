@@ -179,6 +179,16 @@ int hpxfft_exec_r2c_2d_batched(hpxfft_plan plan, void *const *d_grids);
  * that is not batched. */
 int hpxfft_plan_batched_set_config(hpxfft_plan plan, int row_cfg, int col_cfg);
 
+/* No reference counterpart (FFTW's planner choice is internal to
+ * fftw_plan_dft_1d, core/src/util/adapter_fftw.cpp:28-29): runs the plan's
+ * four-step column pass with the split Nx = P * Q, P = 2^log2_p, instead of
+ * the one it chose (estimate: the balanced split; measure and above: the
+ * fastest one timed).  log2_p must lie in the range of splits the kernels
+ * cover (sub-lengths 2^5 .. 2^8).  HPXFFT_INVALID_ARG for another value or
+ * for a plan whose column pass is not a four-step (the plan text says which
+ * it is).  Lets a test check every split the measure effort may keep. */
+int hpxfft_plan_set_column_split(hpxfft_plan plan, int log2_p);
+
 /* Message of the last failing call on this thread ("" if none). */
 const char *hpxfft_last_error(void);
 

@@ -67,6 +67,24 @@ def test_16bit_storage(gpu, dt_name, bound, nx, ny, amp):
     assert err <= bound, err
 
 
+def test_16bit_row_exponents_are_validated(gpu):
+    """The row exponents are read as Nx int32 values on x16's device: any
+    other tensor is refused before a kernel runs."""
+    import torch
+    import hpxfft_amd
+    x = torch.zeros(64, 130, dtype=torch.float16, device="cuda")
+    good = torch.zeros(64, dtype=torch.int32, device="cuda")
+    for bad, err in [(good.cpu(), ValueError), (good.long(), TypeError), (good[:63], ValueError),
+                     (torch.zeros(2, 64, dtype=torch.int32, device="cuda"), ValueError)]:
+        with pytest.raises(err):
+            hpxfft_amd.rfft2_scaled(x, bad)
+    with pytest.raises(ValueError):
+        hpxfft_amd.rfft2_scaled(x.cpu(), None)
+    X16, ce = hpxfft_amd.rfft2_scaled(x, good[::1])
+    torch.cuda.synchronize()
+    assert X16.shape == (64, 65, 2) and not bool(X16.float().abs().sum())
+
+
 @pytest.mark.slow
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 1e-5)])
 def test_headline_grid_vs_oracle(gpu, prec, tol):
