@@ -126,9 +126,17 @@ struct dev_tables
     float *tw32 = nullptr;    // complex float pairs
 };
 
+// The tables live in static device memory: one copy per device, filled from
+// the host by the first get_tables call on that device, shared by every plan
+// and by the allocation-free entry point, held until the process ends.
+constexpr size_t TW_ENTRIES = size_t(2) << TW_MAX_LG;   // entries [N, 2N) for N = 1 .. 2^TW_MAX_LG
+__device__ __align__(256) double tw64_table[2 * TW_ENTRIES];
+__device__ __align__(256) float tw32_table[2 * TW_ENTRIES];
+
 std::mutex g_tab_mu;
 std::map<int, dev_tables> g_tables;
 
+// with device `dev` current
 int get_tables(int dev, dev_tables &out)
 {
     std::lock_guard<std::mutex> lk(g_tab_mu);
@@ -138,7 +146,7 @@ int get_tables(int dev, dev_tables &out)
         out = it->second;
         return HPXFFT_OK;
     }
-    const size_t n = size_t(2) << TW_MAX_LG;   // entries
+    const size_t n = TW_ENTRIES;
     std::vector<double> h64(2 * n, 0.0);
     std::vector<float> h32(2 * n, 0.0f);
     for (int k = 0; k <= TW_MAX_LG; ++k)
@@ -156,16 +164,10 @@ int get_tables(int dev, dev_tables &out)
         }
     }
     dev_tables t;
-    hipError_t e = hipMalloc(&t.tw64, sizeof(double) * 2 * n);
-    if (e == hipSuccess) e = hipMalloc(&t.tw32, sizeof(float) * 2 * n);
-    if (e == hipSuccess) e = hipMemcpy(t.tw64, h64.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(t.tw32, h32.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice);
-    if (e != hipSuccess)
-    {
-        (void)hipFree(t.tw64);
-        (void)hipFree(t.tw32);
-        HIPCHK(e);
-    }
+    HIPCHK(hipGetSymbolAddress(reinterpret_cast<void **>(&t.tw64), HIP_SYMBOL(tw64_table)));
+    HIPCHK(hipGetSymbolAddress(reinterpret_cast<void **>(&t.tw32), HIP_SYMBOL(tw32_table)));
+    HIPCHK(hipMemcpy(t.tw64, h64.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t.tw32, h32.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
     g_tables[dev] = t;
     out = t;
     return HPXFFT_OK;
@@ -263,6 +265,7 @@ struct hpxfft_plan_s
     // timing
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // start, rows done, columns done, column phase A done
     bool ran = false;
+    bool timed = true;           // exec records the phase events (off for the allocation-free entry point)
     bool col_split = false;      // ev[3] valid: last run was a four-step forward
     double h2d = 0.0, d2h = 0.0;
     double plan_seconds = 0.0, plan_flops = 0.0;
@@ -1037,8 +1040,11 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     const bool s16 = p->prec == P_F16 || p->prec == P_BF16;
     if (s16 && !col_exp) col_exp = p->col_exp_own;
     device_guard g(p->device);
-    if (int rc = alloc_events(p)) return rc;
-    HIPCHK(hipEventRecord(p->ev[0], p->stream));
+    if (p->timed)
+    {
+        if (int rc = alloc_events(p)) return rc;
+        HIPCHK(hipEventRecord(p->ev[0], p->stream));
+    }
     RowArgs r{};
     r.in = d_in;
     r.out = d_out;
@@ -1080,10 +1086,11 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
         const unsigned gx = (unsigned)((p->nx + rg.nseq - 1) / rg.nseq);
         HIPCHK(row_r2c(p->prec, p->lgL, r, gx, (unsigned)p->batch, p->stream));
     }
-    HIPCHK(hipEventRecord(p->ev[1], p->stream));
-    if (int rc = run_columns(p, d_out, d_out, false, s16 ? col_exp : nullptr, p->ev[3])) return rc;
-    HIPCHK(hipEventRecord(p->ev[2], p->stream));
-    p->ran = true;
+    if (p->timed) HIPCHK(hipEventRecord(p->ev[1], p->stream));
+    if (int rc = run_columns(p, d_out, d_out, false, s16 ? col_exp : nullptr, p->timed ? p->ev[3] : nullptr))
+        return rc;
+    if (p->timed) HIPCHK(hipEventRecord(p->ev[2], p->stream));
+    p->ran = p->timed;
     p->col_split = p->four_step;
     p->h2d = p->d2h = 0.0;
     return HPXFFT_OK;
@@ -1237,6 +1244,27 @@ int tune_gen_columns(hpxfft_plan_s *p)
     }
     if (rc != HPXFFT_OK) return rc;
     HIPCHK(e);
+    return HPXFFT_OK;
+}
+
+// the power-of-two column pass: one tile of W adjacent columns over all Nx
+// rows, or (Nx above the direct kernels' reach) the four-step Nx = P*Q with
+// the balanced split; sets the four-step scratch size (one P x Q x TW block
+// per strip of TW columns, one slice per grid), allocates nothing
+int choose_columns(hpxfft_plan_s *p)
+{
+    p->W = col_geom(p->prec, p->lgx).nseq;
+    const int dmax = col_direct_max_lg(p->prec);
+    p->four_step = !p->gen_col && p->lgx > dmax;
+    if (!p->four_step) return HPXFFT_OK;
+    p->lgP = p->lgx / 2;
+    p->lgQ = p->lgx - p->lgP;
+    if (p->lgP < COL_TILE_MIN_LG) { p->lgP = COL_TILE_MIN_LG; p->lgQ = p->lgx - p->lgP; }
+    if (p->lgQ > COL_TILE_MAX_LG || p->lgP > COL_TILE_MAX_LG) return fail(HPXFFT_UNSUPPORTED_SIZE, "Nx too large");
+    const size_t cb = compute_bytes(p->prec) * 2;
+    const size_t tw_cols = (size_t)tile_geom(p->prec, p->lgQ).nseq;
+    p->CW = (p->ncols + tw_cols - 1) / tw_cols * tw_cols;
+    p->scratch_bytes = p->nx * p->CW * cb * p->batch;
     return HPXFFT_OK;
 }
 
@@ -1394,23 +1422,8 @@ int plan_common(hpxfft_plan_s *p)
         if (p->g2r.on) HIPCHK(hipMalloc(&p->gtmp, p->batch * p->nx * (p->ny / 2) * cb));
         if (p->g2c.avail) HIPCHK(hipMalloc(&p->gscr, p->batch * p->nx * p->ncols * cb));
     }
-    p->W = col_geom(p->prec, p->lgx).nseq;
-    const int dmax = col_direct_max_lg(p->prec);
-    p->four_step = !p->gen_col && p->lgx > dmax;
-    if (p->four_step)
-    {
-        p->lgP = p->lgx / 2;
-        p->lgQ = p->lgx - p->lgP;
-        if (p->lgP < COL_TILE_MIN_LG) { p->lgP = COL_TILE_MIN_LG; p->lgQ = p->lgx - p->lgP; }
-        if (p->lgQ > COL_TILE_MAX_LG || p->lgP > COL_TILE_MAX_LG)
-            return fail(HPXFFT_UNSUPPORTED_SIZE, "Nx too large");
-        // scratch: one P x Q x TW block per strip of TW columns, one slice per grid
-        const size_t cb = compute_bytes(p->prec) * 2;
-        const size_t tw_cols = (size_t)tile_geom(p->prec, p->lgQ).nseq;
-        p->CW = (p->ncols + tw_cols - 1) / tw_cols * tw_cols;
-        p->scratch_bytes = p->nx * p->CW * cb * p->batch;
-        HIPCHK(hipMalloc(&p->scratch, p->scratch_bytes));
-    }
+    if (int rc = choose_columns(p)) return rc;
+    if (p->four_step) HIPCHK(hipMalloc(&p->scratch, p->scratch_bytes));
     if (p->prec == P_F16 || p->prec == P_BF16)
     {
         HIPCHK(hipMalloc(&p->mid_exp, sizeof(int) * p->nx * p->batch));
@@ -1462,6 +1475,47 @@ void free_plan(hpxfft_plan_s *p)
     (void)hipFree(p->bticket);
     if (p->bstage) (void)hipHostFree(p->bstage);
     delete p;
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (hpxfft_exec_r2c_2d_noalloc): the
+// four-step column pass's intermediate, one array per device, held until the
+// process ends.  Host code never passes this symbol itself to a kernel (on the
+// host it names a stand-in, not device memory): the device address comes from
+// hipGetSymbolAddress, once per device, in noalloc_setup.  Calls that overlap
+// on one device share the array and race; each call rewrites every scratch
+// entry it reads, so nothing needs clearing between calls.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_noalloc_mu;
+std::map<int, void *> g_noalloc_scratch;   // device -> address of its scratch array
+
+// first call on device `dev` (current): the scratch address and the twiddle
+// tables (uploaded once per device, shared with the plans); later calls only
+// look them up
+int noalloc_setup(int dev, hipStream_t stream, void *&scr, dev_tables &tabs)
+{
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    auto it = g_noalloc_scratch.find(dev);
+    if (it == g_noalloc_scratch.end())
+    {
+        bool have_tables;   // a plan on this device may have uploaded them already
+        {
+            std::lock_guard<std::mutex> lt(g_tab_mu);
+            have_tables = g_tables.count(dev) != 0;
+        }
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        HIPCHK(hipStreamIsCapturing(stream, &cs));
+        if (!have_tables && cs != hipStreamCaptureStatusNone)
+            return fail(HPXFFT_INVALID_ARG, "the first allocation-free call on a device uploads its twiddle tables "
+                                            "and cannot be captured: make one call before capturing");
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        it = g_noalloc_scratch.emplace(dev, ptr).first;
+    }
+    scr = it->second;
+    return get_tables(dev, tabs);
 }
 
 }   // namespace
@@ -1547,6 +1601,48 @@ int hpxfft_exec_r2c_2d(hpxfft_plan plan, const void *d_in, void *d_out)
     {
         return fail(HPXFFT_HIP_ERROR, e.what());
     }
+}
+
+int hpxfft_exec_r2c_2d_noalloc(size_t nx, size_t ny, size_t ld_in, size_t ld_out, int precision, const void *d_in,
+                               void *d_out, void *stream)
+{
+    return guarded([&]() -> int {
+        if (precision != HPXFFT_F64 && precision != HPXFFT_F32)
+            return fail(HPXFFT_INVALID_ARG, "the allocation-free entry point takes F64 or F32");
+        const std::string dims = std::to_string(nx) + " x " + std::to_string(ny);
+        if (!is_pow2(nx) || !is_pow2(ny) || ny < 2 || nx > (size_t(1) << (2 * COL_TILE_MAX_LG)) ||
+            ny / 2 > (size_t(1) << ROW_MAX_LG))
+            return fail(HPXFFT_UNSUPPORTED_SIZE, "the allocation-free entry point takes powers of two, Nx <= 65536 "
+                                                 "and 2 <= Ny <= 32768 (got " + dims + ")");
+        if (ld_in < ny || ld_in % 2 || ld_out < ny / 2 + 1)
+            return fail(HPXFFT_INVALID_ARG, "layout: need an even ld_in >= Ny and ld_out >= Ny/2+1");
+        hpxfft_plan_s p;   // the plan's fields only: no events, no device memory of its own
+        p.nx = nx;
+        p.ny = ny;
+        p.ld_in = ld_in;
+        p.dist_in = nx * ld_in;
+        p.ld_out = ld_out;
+        p.dist_out = nx * ld_out;
+        p.prec = precision;
+        p.stream = static_cast<hipStream_t>(stream);
+        p.timed = false;
+        p.lgx = ilog2(nx);
+        p.lgL = ilog2(ny / 2);
+        p.ncols = ny / 2 + 1;
+        if (int rc = check_device(p.device)) return rc;
+        void *scr = nullptr;
+        if (int rc = noalloc_setup(p.device, p.stream, scr, p.tabs)) return rc;
+        if (int rc = choose_columns(&p)) return rc;
+        if (p.four_step && p.scratch_bytes > NOALLOC_SCRATCH_BYTES)
+            return fail(HPXFFT_UNSUPPORTED_SIZE,
+                        "the allocation-free entry point's scratch holds " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                            " bytes; " + dims + " needs " + std::to_string(p.scratch_bytes) +
+                            " (Nx x (Ny/2+1) complex values, rounded to whole column strips)");
+        p.scratch = p.four_step ? scr : nullptr;
+        const int rc = exec_forward(&p, d_in, d_out, nullptr, nullptr);
+        p.scratch = nullptr;
+        return rc;
+    });
 }
 
 int hpxfft_exec_r2c_2d_scaled(hpxfft_plan plan, const void *d_in, void *d_out, const int *d_in_row_exp,

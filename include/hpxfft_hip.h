@@ -100,6 +100,28 @@ int hpxfft_plan_r2c_2d_many(hpxfft_plan *plan, size_t nx, size_t ny, size_t batc
  * for every exec function below). */
 int hpxfft_exec_r2c_2d(hpxfft_plan plan, const void *d_in, void *d_out);
 
+/* Allocation-free form of hpxfft_exec_r2c_2d.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * One grid, Nx x Ny reals at row pitch ld_in in, Nx x (Ny/2+1) complex at
+ * row pitch ld_out out (in place when d_in == d_out and ld_in == 2*ld_out),
+ * F64 or F32, both sides powers of two (Nx <= 65536, 2 <= Ny <= 32768),
+ * planned as hpxfft_plan_r2c_2d at "estimate" on every call and enqueued on
+ * `stream` (a hipStream_t, NULL = default) with no plan object: the results
+ * equal that plan's bit for bit.  No hipMalloc, hipFree or other allocation
+ * happens.  A column pass that runs as a four-step (Nx > 1024) keeps its
+ * intermediate in a static device array of 128 MiB (134217728 bytes) per
+ * device, held until the process ends; a grid whose intermediate,
+ * Nx x (Ny/2+1) complex values of the precision (rounded up to whole strips
+ * of 32 FP64 or 64 FP32 columns), exceeds it is refused with
+ * HPXFFT_UNSUPPORTED_SIZE.  Every call on a device shares that device's array:
+ * calls that overlap on one device (other streams, other threads) race.  The
+ * first call on a device finds the array's address and, unless a plan has
+ * done it already, uploads the twiddle tables: it must not be captured
+ * (HPXFFT_INVALID_ARG if it is); later calls may be. */
+int hpxfft_exec_r2c_2d_noalloc(size_t nx, size_t ny, size_t ld_in, size_t ld_out, int precision, const void *d_in,
+                               void *d_out, void *stream);
+
 /* 16-bit storage plans: value = stored * 2^-exp.  d_in_row_exp (optional,
  * Nx ints per grid) scales the input rows; d_out_col_exp (required, Ny/2+1
  * ints per grid) receives the per-column exponent of the output (the second
