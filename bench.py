@@ -13,7 +13,12 @@ Prints ONE JSON line.  Flags: --steps K --warmup W --gpus N (N must be 1)
 --no-cpu (skip the CPU baseline leg) --no-f32 --no-extra (skip the other
 legs) --nx/--ny (other sizes) --copy-probe BYTES (only a device copy, for
 calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE) --general-only (only the
-general-length legs, one JSON line).
+general-length legs, one JSON line) --only LEG (one leg by name, for
+profiling it alone).
+
+Every leg reports its launches (event-timed seconds, algorithmic bytes and,
+where a committed counter summary covers the workload, HBM traffic) and a
+roofline entry for its longest launch.
 
 roofline: the dominant kernel, the pass with the longest event-timed
 duration on the plan's stream (a four-step column pass is one kernel in two
@@ -85,13 +90,7 @@ def time_gpu(nx: int, ny: int, precision: str, steps: int, warmup: int, seed: in
     wall = time.perf_counter() - t0
     ms = e0.elapsed_time(e1) / steps
     # per-phase times from the plan's own HIP events (row pass, column pass)
-    keys = ("first_fftw", "second_fftw", "column_a", "column_b")
-    ph = {k: [] for k in keys}
-    for _ in range(max(3, min(steps, 10))):
-        plan.exec(x.data_ptr(), out.data_ptr())
-        for k in keys:
-            ph[k].append(plan.measurement(k))
-    med = {k: sorted(v)[len(v) // 2] for k, v in ph.items()}
+    med = phase_medians(plan, lambda: plan.exec(x.data_ptr(), out.data_ptr()), max(3, min(steps, 10)))
     desc = plan.describe()
     plan_s = plan.measurement("plan")
     plan.close()
@@ -124,6 +123,53 @@ def measured_traffic(workload: str) -> dict:
             return json.load(f).get(workload, {})
     except (OSError, ValueError):
         return {}
+
+
+def phase_medians(plan, run, reps: int = 5) -> dict:
+    """Median event-timed seconds of each phase key over `reps` runs of `run()`
+    (one exec of `plan` on the plan's stream)."""
+    keys = ("first_fftw", "second_fftw", "column_a", "column_b")
+    ph = {k: [] for k in keys}
+    for _ in range(reps):
+        run()
+        for k in keys:
+            ph[k].append(plan.measurement(k))
+    return {k: sorted(v)[len(v) // 2] for k, v in ph.items()}
+
+
+def pass_launches(med: dict, row_bytes: int, col_bytes: int, backward: bool = False, row="row_r2c") -> dict:
+    """Per-launch (algorithmic bytes, seconds) of a forward (row pass, then
+    column pass) or backward (column pass, then rows) exec.  A four-step
+    column pass is two launches, phases A and B, credited half the column
+    pass's bytes each."""
+    row_s, col_s = (med["second_fftw"], med["first_fftw"]) if backward else (med["first_fftw"], med["second_fftw"])
+    out = {row: (row_bytes, row_s)}
+    if med["column_a"] > 0.0:
+        out["column_phase_a"] = (col_bytes // 2, med["column_a"])
+        out["column_phase_b"] = (col_bytes // 2, med["column_b"])
+    else:
+        out["column_c2c"] = (col_bytes, col_s)
+    return out
+
+
+def launch_table(launches: dict, workload: str) -> dict:
+    tr = measured_traffic(workload)
+    return {k: {"s": v[1], "alg_bytes": v[0], "alg_GBps": round(v[0] / v[1] / 1e9, 1) if v[1] > 0 else None,
+                "traffic": tr.get(k),
+                "traffic_GBps": round(tr[k] / v[1] / 1e9, 1) if tr.get(k) and v[1] > 0 else None}
+            for k, v in launches.items()}
+
+
+def leg_roofline(launches: dict, workload: str) -> dict:
+    """Roofline entry of a leg's dominant launch (the longest event-timed one):
+    its algorithmic bytes / its duration against the HBM peak, and its HBM
+    traffic from the committed counter summary when one covers the workload."""
+    name = max(launches, key=lambda k: launches[k][1])
+    b, sec = launches[name]
+    ach = b / sec / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_traffic(workload).get(name), "kernel": name,
+            "alg_bytes_per_launch": b, "avg_launch_s": sec, "workload": workload}
 
 
 def progress(msg: str) -> None:
@@ -187,6 +233,7 @@ def bench_batched(count: int = 512, seed: int = 4, reps: int = 5) -> dict:
     default = hpxfft_amd.BatchedPlan(nx, ny, effort="estimate", stream=stream)
     t_tuned = _events_ms(lambda: tuned.exec(ptrs), reps)
     t_default = _events_ms(lambda: default.exec(ptrs), reps)
+    med = phase_medians(tuned, lambda: tuned.exec(ptrs), reps)
     singles = [hpxfft_amd.Plan(a, b, precision="f32", stream=stream) for a, b in zip(nx, ny)]
 
     def per_grid():
@@ -195,38 +242,58 @@ def bench_batched(count: int = 512, seed: int = 4, reps: int = 5) -> dict:
 
     t_single = _events_ms(per_grid, reps)
     alg = sum(a * b * 4 + a * (b // 2 + 1) * 8 for a, b in zip(nx, ny))   # 1-pass compulsory model
+    H = sum(a * (b // 2 + 1) * 8 for a, b in zip(nx, ny))
+    launches = {"batched_rows": (alg, med["first_fftw"]), "batched_columns": (2 * H, med["second_fftw"])}
     desc = tuned.describe()
     for p in singles:
         p.close()
     tuned.close()
     default.close()
+    wl = f"batched_{count}_seed{seed}"
     return {"grids": count, "seed": seed, "alg_bytes_1pass": alg, "ms_autotuned": round(t_tuned, 4),
             "ms_default_tiles": round(t_default, 4), "ms_per_grid_launches": round(t_single, 4),
             "speedup_vs_per_grid": round(t_single / t_tuned, 3), "GBps_1pass": round(alg / (t_tuned * 1e-3) / 1e9, 1),
+            "hbm_frac_1pass": round(alg / (t_tuned * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "launches": launch_table(launches, wl), "roofline": leg_roofline(launches, wl),
             "plan": desc.strip().splitlines()[1:3]}
 
 
 def bench_adjoint(batch: int = 16, n: int = 4096, reps: int = 5) -> dict:
-    """Differentiable workload: batch 16 x 2^12 x 2^12 FP32, forward and adjoint."""
+    """Differentiable workload: batch 16 x 2^12 x 2^12 FP32, forward and
+    adjoint, through one plan of 16 grids (the torch binding's layout)."""
     import torch
     import hpxfft_amd
 
     g = torch.Generator(device="cuda").manual_seed(5)
     x = torch.randn(batch, n, n, dtype=torch.float32, device="cuda", generator=g)
-    hpxfft_amd.set_planning_effort("measure")   # as the headline: planned once, outside the timed region
-    X = hpxfft_amd.rfft2(x)
-    hpxfft_amd.rfft2_adjoint(X, n)
-    t_f = _events_ms(lambda: hpxfft_amd.rfft2(x), reps)
-    t_a = _events_ms(lambda: hpxfft_amd.rfft2_adjoint(X, n), reps)
-    hpxfft_amd.set_planning_effort("estimate")
-    hpxfft_amd.clear_plan_cache()
-    B = batch * alg_bytes(n, n, 4)["total"]
+    X = torch.empty(batch, n, n // 2 + 1, dtype=torch.complex64, device="cuda")
+    r = torch.empty_like(x)
+    p = hpxfft_amd.Plan(n, n, batch=batch, ld_in=n, ld_out=n // 2 + 1, precision="f32", effort="measure",
+                        stream=torch.cuda.current_stream())
+    fwd = lambda: p.exec(x.data_ptr(), X.data_ptr())   # noqa: E731
+    adj = lambda: p.exec_adjoint(X.data_ptr(), r.data_ptr())   # noqa: E731
+    fwd()
+    adj()
+    t_f = _events_ms(fwd, reps)
+    med_f = phase_medians(p, fwd, reps)
+    t_a = _events_ms(adj, reps)
+    med_a = phase_medians(p, adj, reps)
+    p.close()
+    b1 = alg_bytes(n, n, 4)
+    B = batch * b1["total"]
+    lf = pass_launches(med_f, batch * b1["row"], batch * b1["col"])
+    la = pass_launches(med_a, batch * b1["row"], batch * b1["col"], backward=True, row="row_c2r")
+    wl = f"adjoint_{batch}x{n}x{n}"
     return {"config": f"{batch} x {n} x {n} FP32", "fwd_ms": round(t_f, 4), "adjoint_ms": round(t_a, 4),
-            "fwd_GBps": round(B / (t_f * 1e-3) / 1e9, 1), "adjoint_GBps": round(B / (t_a * 1e-3) / 1e9, 1)}
+            "fwd_GBps": round(B / (t_f * 1e-3) / 1e9, 1), "adjoint_GBps": round(B / (t_a * 1e-3) / 1e9, 1),
+            "launches_forward": launch_table(lf, wl + "_fwd"), "launches_adjoint": launch_table(la, wl + "_adj"),
+            "roofline_forward": leg_roofline(lf, wl + "_fwd"), "roofline": leg_roofline(la, wl + "_adj")}
 
 
 def bench_mixed(n: int = 1 << 15, reps: int = 3) -> dict:
-    """2^15 x 2^15 N(0,1): FP32, FP16-storage and BF16-storage rel-L2 vs the FP64 path, and times."""
+    """2^15 x 2^15 N(0,1): FP64, FP32, FP16-storage and BF16-storage times,
+    per-launch event timings and roofline entries, and the rel-L2 of each
+    against the FP64 path (which the oracle pins at this size)."""
     import torch
     import hpxfft_amd
 
@@ -238,38 +305,55 @@ def bench_mixed(n: int = 1 << 15, reps: int = 3) -> dict:
     out = {}
     ref = torch.empty((n, n // 2 + 1), dtype=torch.complex128, device="cuda")
     p64 = hpxfft_amd.Plan(n, n, ld_in=ld, precision="f64", effort="measure")
-    p64.exec(x64.data_ptr(), ref.data_ptr())
-    out["f64_ms"] = round(_events_ms(lambda: p64.exec(x64.data_ptr(), ref.data_ptr()), reps), 3)
+    run64 = lambda: p64.exec(x64.data_ptr(), ref.data_ptr())   # noqa: E731
+    run64()
+    t64 = _events_ms(run64, reps)
+    med = phase_medians(p64, run64, reps)
+    b = alg_bytes(n, n, 8)
+    l64 = pass_launches(med, b["row"], b["col"])
+    out["f64"] = {"ms": round(t64, 3), "GBps": round(b["total"] / (t64 * 1e-3) / 1e9, 1), "column_pass": _split_line(p64),
+                  "launches": launch_table(l64, f"f64_{n}x{n}"), "roofline": leg_roofline(l64, f"f64_{n}x{n}")}
     p64.close()
     nref = torch.linalg.vector_norm(ref)
     x32 = x64.float()
     y32 = torch.empty((n, n // 2 + 1), dtype=torch.complex64, device="cuda")
     p32 = hpxfft_amd.Plan(n, n, ld_in=ld, precision="f32", effort="measure")
-    t32 = _events_ms(lambda: p32.exec(x32.data_ptr(), y32.data_ptr()), reps)
+    run32 = lambda: p32.exec(x32.data_ptr(), y32.data_ptr())   # noqa: E731
+    t32 = _events_ms(run32, reps)
+    med = phase_medians(p32, run32, reps)
+    b = alg_bytes(n, n, 4)
+    l32 = pass_launches(med, b["row"], b["col"])
     split32 = _split_line(p32)
     p32.close()
     err = (torch.linalg.vector_norm(y32.to(torch.complex128) - ref) / nref).item()
-    out["f32"] = {"ms": round(t32, 3), "rel_l2_vs_f64": err,
-                  "GBps": round(alg_bytes(n, n, 4)["total"] / (t32 * 1e-3) / 1e9, 1), "column_pass": split32}
+    out["f32"] = {"ms": round(t32, 3), "rel_l2_vs_f64": err, "GBps": round(b["total"] / (t32 * 1e-3) / 1e9, 1),
+                  "column_pass": split32, "launches": launch_table(l32, f"f32_{n}x{n}"),
+                  "roofline": leg_roofline(l32, f"f32_{n}x{n}")}
     del x32, y32
     for name, dt, prec in (("f16", torch.float16, "f16"), ("bf16", torch.bfloat16, "bf16")):
         x16 = x64.to(dt)
         buf = torch.empty_like(x16)
         col_exp = torch.empty(n // 2 + 1, dtype=torch.int32, device="cuda")
         p = hpxfft_amd.Plan(n, n, ld_in=ld, ld_out=ld // 2, precision=prec, effort="measure")
+        run = lambda: (buf.copy_(x16), p.exec_scaled(buf.data_ptr(), buf.data_ptr(), None,   # noqa: E731
+                                                     col_exp.data_ptr()))
         # the copy alone first: the timed loop must be the last writer of buf
         tc = _events_ms(lambda: buf.copy_(x16), reps)
-        t = _events_ms(lambda: (buf.copy_(x16), p.exec_scaled(buf.data_ptr(), buf.data_ptr(), None,
-                                                              col_exp.data_ptr())), reps)
+        t = _events_ms(run, reps)
+        med = phase_medians(p, run, reps)
+        b = alg_bytes(n, n, 2)
+        l16 = pass_launches(med, b["row"], b["col"])
         split = _split_line(p)
         p.close()
         Xs = buf.view(n, ld // 2, 2).float()
         X = torch.complex(Xs[..., 0], Xs[..., 1]).to(torch.complex128) * torch.exp2(-col_exp.double())[None, :]
         e = (torch.linalg.vector_norm(X - ref) / nref).item()
         out[name] = {"ms": round(t - tc, 3), "rel_l2_vs_f64": e,
-                     "GBps": round(alg_bytes(n, n, 2)["total"] / ((t - tc) * 1e-3) / 1e9, 1),
+                     "GBps": round(b["total"] / ((t - tc) * 1e-3) / 1e9, 1),
+                     "vs_f32_time": round((t - tc) / t32, 3),
                      "scales": "per input row after the row pass, per output column after the column pass",
-                     "column_pass": split}
+                     "column_pass": split, "launches": launch_table(l16, f"{name}_{n}x{n}"),
+                     "roofline": leg_roofline(l16, f"{name}_{n}x{n}")}
         del x16, buf, Xs, X
     return out
 
@@ -295,23 +379,27 @@ def bench_c1(nx: int = 1 << 8, ny: int = 1 << 14, seed: int = 1, reps: int = 50,
     out = torch.empty((nx, ny // 2 + 1), dtype=torch.complex128, device="cuda")
     s = torch.cuda.Stream()
     p = hpxfft_amd.Plan(nx, ny, ld_in=ld, precision="f64", effort="estimate", stream=s)
+    run = lambda: p.exec(x.data_ptr(), out.data_ptr())   # noqa: E731
     with torch.cuda.stream(s):
-        eager = _events_ms(lambda: p.exec(x.data_ptr(), out.data_ptr()), reps, warmup=5)
+        eager = _events_ms(run, reps, warmup=5)
+        med = phase_medians(p, run, 9)
         graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(graph, stream=s):
             for _ in range(per_graph):
-                p.exec(x.data_ptr(), out.data_ptr())
+                run()
         graphed = _events_ms(graph.replay, max(3, reps // per_graph), warmup=2) / per_graph
-        row_s, col_s = p.measurement("first_fftw"), p.measurement("second_fftw")
     s.synchronize()
     desc = [ln.strip() for ln in p.describe().splitlines() if "kernel" in ln]
     p.close()
-    B = alg_bytes(nx, ny, 8)["total"]
-    return {"config": f"{nx}x{ny} FP64 U[0,1) seed {seed}, plan estimate", "alg_bytes": B,
+    b = alg_bytes(nx, ny, 8)
+    launches = pass_launches(med, b["row"], b["col"])
+    wl = f"f64_{nx}x{ny}"
+    return {"config": f"{nx}x{ny} FP64 U[0,1) seed {seed}, plan estimate", "alg_bytes": b["total"],
             "ms_eager": round(eager, 5), "ms_graph": round(graphed, 5),
-            "GBps_eager": round(B / (eager * 1e-3) / 1e9, 1), "GBps_graph": round(B / (graphed * 1e-3) / 1e9, 1),
-            "hbm_frac_graph": round(B / (graphed * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-            "phases_s_last_eager_exec": {"row_r2c": row_s, "column_c2c": col_s}, "kernels": desc}
+            "GBps_eager": round(b["total"] / (eager * 1e-3) / 1e9, 1),
+            "GBps_graph": round(b["total"] / (graphed * 1e-3) / 1e9, 1),
+            "hbm_frac_graph": round(b["total"] / (graphed * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "launches": launch_table(launches, wl), "roofline": leg_roofline(launches, wl), "kernels": desc}
 
 
 def bench_general(nx: int = 3000, ny: int = 3000, reps: int = 10) -> dict:
@@ -319,7 +407,8 @@ def bench_general(nx: int = 3000, ny: int = 3000, reps: int = 10) -> dict:
     (2-pass byte model): by default 3000 x 3000 (radices 4, 2, 3, 5); with
     prime lengths such as 2039 x 4078 the Bluestein path; 6000 x 12000 runs
     its long lengths (radices 2..7) in place in one LDS buffer, and lengths
-    beyond that such as 9000 x 18000 take the two-level path."""
+    beyond that such as 9000 x 18000 take the two-level path (two launches
+    per pass: the roofline entry is then per pass)."""
     import torch
     import hpxfft_amd
 
@@ -330,12 +419,16 @@ def bench_general(nx: int = 3000, ny: int = 3000, reps: int = 10) -> dict:
     x[:, ny:].zero_()
     out = torch.empty((nx, ny // 2 + 1), dtype=torch.complex128, device="cuda")
     p = hpxfft_amd.Plan(nx, ny, ld_in=ld, precision="f64", effort="measure")
-    ms = _events_ms(lambda: p.exec(x.data_ptr(), out.data_ptr()), reps)
+    run = lambda: p.exec(x.data_ptr(), out.data_ptr())   # noqa: E731
+    ms = _events_ms(run, reps)
+    med = phase_medians(p, run, 5)
     desc = [ln.strip() for ln in p.describe().splitlines() if "general" in ln]
     p.close()
-    B = alg_bytes(nx, ny, 8)["total"]
-    return {"config": f"{nx}x{ny} FP64", "ms": round(ms, 4), "GBps": round(B / (ms * 1e-3) / 1e9, 1),
-            "kernels": desc}
+    b = alg_bytes(nx, ny, 8)
+    launches = {"row_pass": (b["row"], med["first_fftw"]), "column_pass": (b["col"], med["second_fftw"])}
+    wl = f"f64_{nx}x{ny}"
+    return {"config": f"{nx}x{ny} FP64", "ms": round(ms, 4), "GBps": round(b["total"] / (ms * 1e-3) / 1e9, 1),
+            "launches": launch_table(launches, wl), "roofline": leg_roofline(launches, wl), "kernels": desc}
 
 
 def general_legs() -> dict:
@@ -371,6 +464,29 @@ def cpu_baseline(nx: int, ny: int, budget_s: float = 20.0):
             "seconds": t, "phases": dict(loop.measurements)}
 
 
+def _headline_leg(precision: str, steps: int = 5, warmup: int = 2) -> dict:
+    n = 1 << 14
+    r = time_gpu(n, n, precision, steps, warmup, seed=2)
+    s = 8 if precision == "f64" else 4
+    lt = launches(n, n, s, r)
+    wl = f"{precision}_{n}x{n}"
+    return {"ms_per_step": round(r["ms_per_step"], 4), "launches": launch_table(lt, wl),
+            "roofline": leg_roofline(lt, wl)}
+
+
+# the legs --only can run on their own (each prints the object the full run nests under its name)
+LEGS = {
+    "f64_16384x16384": lambda: _headline_leg("f64"),
+    "f32_16384x16384": lambda: _headline_leg("f32"),
+    "readme_shape_2e8x2e14": bench_c1,
+    "batched_uneven": bench_batched,
+    "adjoint": bench_adjoint,
+    "mixed_2e15": bench_mixed,
+    "general_length": bench_general,
+    "general_length_bluestein": lambda: bench_general(2039, 2 * 2039),
+}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -385,6 +501,8 @@ def main() -> int:
                     help="only the lengths that are not powers of two (FP64, plans at measure): one JSON line")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (reads BYTES, writes BYTES) three times")
+    ap.add_argument("--only", choices=sorted(LEGS), default=None,
+                    help="run one leg and print its JSON object (for profiling a leg on its own)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the 2-D r2c path runs on a single MI355X (no multi-GPU path is built)",
@@ -399,6 +517,9 @@ def main() -> int:
         return copy_probe(args.copy_probe)
     if args.general_only:
         print(json.dumps(general_legs()), flush=True)
+        return 0
+    if args.only:
+        print(json.dumps({args.only: LEGS[args.only]()}), flush=True)
         return 0
     nx, ny = args.nx, args.ny
     r64 = time_gpu(nx, ny, "f64", args.steps, args.warmup, seed=2)
@@ -472,12 +593,15 @@ def main() -> int:
         err = (torch.linalg.vector_norm(out32.to(torch.complex128) - ref) / torch.linalg.vector_norm(ref)).item()
         p32.close()
         p64.close()
+        l32 = launches(nx, ny, 4, r32)
         result["f32"] = {"ms_per_step": round(r32["ms_per_step"], 4),
                          "GBps": round(b32["total"] / (r32["ms_per_step"] * 1e-3) / 1e9, 2),
                          "speedup_vs_f64": round(r64["ms_per_step"] / r32["ms_per_step"], 3),
                          "rel_l2_vs_f64": err,
                          "gflops": round(nominal_flops(nx, ny) / (r32["ms_per_step"] * 1e-3) / 1e9, 1),
-                         "phases_s": {"row_r2c": r32["row_s"], "column_c2c": r32["col_s"]}}
+                         "phases_s": {"row_r2c": r32["row_s"], "column_c2c": r32["col_s"]},
+                         "launches": launch_table(l32, f"f32_{nx}x{ny}"),
+                         "roofline": leg_roofline(l32, f"f32_{nx}x{ny}")}
     if not args.no_extra:
         torch.cuda.empty_cache()
         result["readme_shape_2e8x2e14"] = bench_c1()

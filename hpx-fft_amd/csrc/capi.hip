@@ -266,7 +266,8 @@ struct hpxfft_plan_s
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // start, rows done, columns done, column phase A done
     bool ran = false;
     bool timed = true;           // exec records the phase events (off for the allocation-free entry point)
-    bool col_split = false;      // ev[3] valid: last run was a four-step forward
+    bool col_split = false;      // ev[3] valid: the last run's column pass was a four-step
+    bool col_first = false;      // the last run was backward: column pass (ev0..ev1) before the rows
     double h2d = 0.0, d2h = 0.0;
     double plan_seconds = 0.0, plan_flops = 0.0;
     std::map<std::string, double> extra;
@@ -282,7 +283,10 @@ struct hpxfft_plan_s
     double btune_ms[2][hpxfft_host::BATCH_NCFG] = {};
     double split_ms[COL_TILE_MAX_LG + 1] = {};   // four-step: column-pass ms per tried log2 P (effort >= measure)
     unsigned *bticket = nullptr;                            // two work-queue counters
-    void *bstage = nullptr;                                 // pinned staging of the grid descriptors
+    void *bstage = nullptr;                                 // pinned staging of the grid descriptors, two slots
+    hipEvent_t bstage_ev[2] = {nullptr, nullptr};           // each slot's last upload (the slot is free once it ran)
+    int bstage_next = 0;                                    // slot of the next upload
+    std::vector<void *> bptrs;                              // grid pointers the device descriptors hold now
 };
 
 namespace {
@@ -1092,6 +1096,7 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     if (p->timed) HIPCHK(hipEventRecord(p->ev[2], p->stream));
     p->ran = p->timed;
     p->col_split = p->four_step;
+    p->col_first = false;
     p->h2d = p->d2h = 0.0;
     return HPXFFT_OK;
 }
@@ -1473,6 +1478,8 @@ void free_plan(hpxfft_plan_s *p)
     for (auto &ph : p->btasks)
         for (auto &t : ph) (void)hipFree(t);
     (void)hipFree(p->bticket);
+    for (auto &e : p->bstage_ev)
+        if (e) (void)hipEventDestroy(e);
     if (p->bstage) (void)hipHostFree(p->bstage);
     delete p;
 }
@@ -1719,7 +1726,7 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     }
     if (int rc = alloc_events(plan)) return rc;
     HIPCHK(hipEventRecord(plan->ev[0], plan->stream));
-    if (int rc = run_columns(plan, d_spec, plan->adj_work, true, nullptr)) return rc;
+    if (int rc = run_columns(plan, d_spec, plan->adj_work, true, nullptr, plan->ev[3])) return rc;
     HIPCHK(hipEventRecord(plan->ev[1], plan->stream));
     RowArgs r{};
     r.in = plan->adj_work;
@@ -1765,7 +1772,8 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     }
     HIPCHK(hipEventRecord(plan->ev[2], plan->stream));
     plan->ran = true;
-    plan->col_split = false;
+    plan->col_split = plan->four_step;
+    plan->col_first = true;
     return HPXFFT_OK;
 }
 
@@ -1816,7 +1824,11 @@ int hpxfft_get_measurement(hpxfft_plan plan, const char *key, double *value)
             {
                 device_guard g(plan->device);
                 HIPCHK(hipEventSynchronize(plan->ev[2]));
-                v = k == "column_a" ? elapsed(plan->ev[1], plan->ev[3]) : elapsed(plan->ev[3], plan->ev[2]);
+                // forward: rows ev0..ev1, columns ev1..ev3..ev2; backward: columns ev0..ev3..ev1, rows ev1..ev2
+                if (plan->col_first)
+                    v = k == "column_a" ? elapsed(plan->ev[0], plan->ev[3]) : elapsed(plan->ev[3], plan->ev[1]);
+                else
+                    v = k == "column_a" ? elapsed(plan->ev[1], plan->ev[3]) : elapsed(plan->ev[3], plan->ev[2]);
             }
         }
         else
@@ -1888,17 +1900,36 @@ int batched_run(hpxfft_plan_s *p, int cfg_row, int cfg_col)
     return HPXFFT_OK;
 }
 
+// upload the grid descriptors for d_grids unless the device already holds
+// them (the same pointers as the last upload): then nothing is enqueued.  The
+// pinned staging has two slots used in turn; a slot is rewritten only after
+// the copy that last read it has run, so no call waits for the stream.
 int batched_upload_grids(hpxfft_plan_s *p, void *const *d_grids)
 {
-    const size_t gb = batched_grid_desc_bytes();
-    char *st = reinterpret_cast<char *>(p->bstage);
-    for (size_t i = 0; i < p->bnx.size(); ++i)
+    const size_t n = p->bnx.size(), gb = batched_grid_desc_bytes();
+    for (size_t i = 0; i < n; ++i)
     {
         if (!d_grids[i]) return fail(HPXFFT_INVALID_ARG, "null grid pointer " + std::to_string(i));
         if (int rc = check_aligned(p, d_grids[i], d_grids[i])) return rc;
-        batched_fill_grid(st + i * gb, reinterpret_cast<float *>(d_grids[i]), (int)p->bnx[i], (int)p->bny[i]);
     }
-    HIPCHK(hipMemcpyAsync(p->bgrids, p->bstage, gb * p->bnx.size(), hipMemcpyHostToDevice, p->stream));
+    if (p->bptrs.size() == n && std::equal(p->bptrs.begin(), p->bptrs.end(), d_grids)) return HPXFFT_OK;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIPCHK(hipStreamIsCapturing(p->stream, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+        return fail(HPXFFT_INVALID_ARG, "a captured batched exec must use the grid pointers of the plan's previous "
+                                        "exec (a new pointer set is uploaded from host memory, which a graph "
+                                        "cannot replay)");
+    const int slot = p->bstage_next;
+    HIPCHK(hipEventSynchronize(p->bstage_ev[slot]));
+    char *st = reinterpret_cast<char *>(p->bstage) + (size_t)slot * gb * n;
+    for (size_t i = 0; i < n; ++i)
+        batched_fill_grid(st + i * gb, reinterpret_cast<float *>(d_grids[i]), (int)p->bnx[i], (int)p->bny[i]);
+    // forget the old set first: if the copy fails, the next call uploads again
+    p->bptrs.clear();
+    HIPCHK(hipMemcpyAsync(p->bgrids, st, gb * n, hipMemcpyHostToDevice, p->stream));
+    HIPCHK(hipEventRecord(p->bstage_ev[slot], p->stream));
+    p->bstage_next = 1 - slot;
+    p->bptrs.assign(d_grids, d_grids + n);
     return HPXFFT_OK;
 }
 
@@ -1954,6 +1985,7 @@ int batched_autotune(hpxfft_plan_s *p)
     if (b) (void)hipEventDestroy(b);
     (void)hipStreamSynchronize(p->stream);
     (void)hipFree(buf);
+    p->bptrs.clear();   // the descriptors point into the freed buffer: the next exec uploads
     return rc;
 }
 
@@ -1973,7 +2005,12 @@ int plan_batched(hpxfft_plan_s *p)
     if (int rc = alloc_events(p)) return rc;
     const size_t n = p->bnx.size();
     HIPCHK(hipMalloc(&p->bgrids, batched_grid_desc_bytes() * n));
-    HIPCHK(hipHostMalloc(&p->bstage, batched_grid_desc_bytes() * n, hipHostMallocDefault));
+    HIPCHK(hipHostMalloc(&p->bstage, 2 * batched_grid_desc_bytes() * n, hipHostMallocDefault));
+    for (auto &e : p->bstage_ev)
+    {
+        HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(e, p->stream));   // both slots start free
+    }
     HIPCHK(hipMalloc(&p->bticket, 2 * sizeof(unsigned)));
     const size_t tb = batched_task_desc_bytes();
     for (int ph = 0; ph < 2; ++ph)
@@ -2100,8 +2137,6 @@ int hpxfft_exec_r2c_2d_batched(hpxfft_plan plan, void *const *d_grids)
         if (!plan || !plan->batched) return fail(HPXFFT_INVALID_ARG, "not a batched plan");
         if (!d_grids) return fail(HPXFFT_INVALID_ARG, "null grid pointer array");
         device_guard g(plan->device);
-        // the pinned descriptor staging buffer may still be read by a previous upload
-        HIPCHK(hipStreamSynchronize(plan->stream));
         if (int rc = batched_upload_grids(plan, d_grids)) return rc;
         HIPCHK(hipEventRecord(plan->ev[0], plan->stream));
         if (int rc = batched_run(plan, plan->bcfg[0], plan->bcfg[1])) return rc;

@@ -121,20 +121,29 @@ __device__ __forceinline__ void row_r2c_body(const RowArgs &a, long long b, int 
     SO *out = reinterpret_cast<SO *>(a.out) + b * a.dist_out;
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(a.tw);
 
+    // loads from in-bounds rows only, unguarded (rows past the end read the
+    // last row and take 0), so all E are in flight before the first wait
     cpx<T> v[E];
 #pragma unroll
     for (int x = 0; x < E; ++x)
     {
         int seq, e;
         EN::in_elem(t, x, seq, e);
-        const int row = row0 + seq;
-        cpx<T> z = {T(0), T(0)};
-        if (row < a.nrows)
-        {
-            z = stor<SI>::ld(in + (long long)row * a.ld_in + 2 * e);
-            if (a.in_exp) { const T s = (T)ldexp(1.0, -a.in_exp[b * a.exp_dist + row]); z = scal(z, s); }
-        }
+        const int row = row0 + seq, rowc = row < a.nrows ? row : a.nrows - 1;
+        cpx<T> z = stor<SI>::ld(in + (long long)rowc * a.ld_in + 2 * e);
+        if (row >= a.nrows) z = cpx<T>{T(0), T(0)};
         v[x] = z;
+    }
+    if (a.in_exp)
+    {
+#pragma unroll
+        for (int x = 0; x < E; ++x)
+        {
+            int seq, e;
+            EN::in_elem(t, x, seq, e);
+            const int row = row0 + seq, rowc = row < a.nrows ? row : a.nrows - 1;
+            v[x] = scal(v[x], (T)ldexp(1.0, -a.in_exp[b * a.exp_dist + rowc]));
+        }
     }
 
     EN::run(v, lds, tw, 2, t);
@@ -281,23 +290,32 @@ __device__ __forceinline__ void col_body(const ColArgs &a, long long b, int stri
     const SI *in = reinterpret_cast<const SI *>(a.in) + 2 * b * a.dist;   // dist counts complex elements
     SO *out = reinterpret_cast<SO *>(a.out) + 2 * b * a.dist;
 
+    // unguarded loads from in-bounds columns (lanes past the last column read
+    // it and take 0): all E in flight before the first wait
     cpx<T> v[E];
 #pragma unroll
     for (int x = 0; x < E; ++x)
     {
         int w, e;
         EN::in_elem(t, x, w, e);
-        cpx<T> z = {T(0), T(0)};
-        if (w < nvalid)
-        {
-            z = stor<SI>::ld(in + 2 * ((long long)e * a.pitch + col0 + w));
-            if constexpr (is16<SI>())
-            {
-                if (a.row_exp) z = scal(z, (T)ldexpf(1.0f, -a.row_exp[b * a.rexp_dist + e]));
-            }
-            if constexpr (CONJ) z = conjc(z);
-        }
+        const int wc = w < nvalid ? w : nvalid - 1;
+        cpx<T> z = stor<SI>::ld(in + 2 * ((long long)e * a.pitch + col0 + wc));
+        if (w >= nvalid) z = cpx<T>{T(0), T(0)};
+        if constexpr (CONJ) z = conjc(z);
         v[x] = z;
+    }
+    if constexpr (is16<SI>())
+    {
+        if (a.row_exp)
+        {
+#pragma unroll
+            for (int x = 0; x < E; ++x)
+            {
+                int w, e;
+                EN::in_elem(t, x, w, e);
+                v[x] = scal(v[x], (T)ldexpf(1.0f, -a.row_exp[b * a.rexp_dist + e]));
+            }
+        }
     }
 
     EN::run(v, lds, reinterpret_cast<const cpx<T> *>(a.tw_L), 1, t);

@@ -156,30 +156,45 @@ tile_kernel(ColArgs a)
     const long long sbase = (long long)strip * a.P * a.Q * W;   // phase A's scratch block (within the slice)
     const cpx<T> *twL = reinterpret_cast<const cpx<T> *>(a.tw_L);
 
-    // ---- load: element e = jj + r*JPC of column w (stage-0 butterfly jj)
+    // ---- load: element e = jj + r*JPC of column w (stage-0 butterfly jj).
+    // Lanes outside the grid load from an in-bounds column and then take 0:
+    // no branch around the loads, so all 16 are in flight before the first
+    // wait (a guarded load followed by a conversion cost one round trip each)
+    const int cl = valid ? c : (c < 0 ? 0 : a.ncols - 1);
     cpx<T> v[16];
 #pragma unroll
     for (int r = 0; r < 16; ++r)
     {
         const int e = jj + r * JPC;
         long long idx;
-        if constexpr (MODE == 1) idx = (long long)(sub + a.P * e) * a.pitch + c;
+        if constexpr (MODE == 1) idx = (long long)(sub + a.P * e) * a.pitch + cl;
         else
         {
-            const int blk = valid ? c / W : 0;   // scratch block holding column c
-            idx = (((long long)blk * a.Q + sub) * a.P + e) * W + (c - blk * W);
+            const int blk = cl / W;   // scratch block holding column cl
+            idx = (((long long)blk * a.Q + sub) * a.P + e) * W + (cl - blk * W);
         }
-        cpx<T> z = {T(0), T(0)};
-        if (valid)
-        {
-            z = stor<SIN>::ld(gin + 2 * idx);
-            if constexpr (S16 && MODE == 1)
-            {
-                if (a.row_exp) z = scal(z, (T)ldexpf(1.0f, -a.row_exp[gb * a.rexp_dist + sub + a.P * e]));
-            }
-        }
+        cpx<T> z = stor<SIN>::ld(gin + 2 * idx);
+        if (!valid) z = cpx<T>{T(0), T(0)};
         if constexpr (CONJ && MODE == 1) z = conjc(z);
         v[r] = z;
+    }
+    if constexpr (S16 && MODE == 1)
+    {
+        // unscale row m1 + P e by 2^-row_exp.  W is a multiple of 64, so jj
+        // is the same on every lane of a wave: the 16 exponents are
+        // wave-uniform reads, issued after (and waited for with) the data
+        // loads rather than one round trip per element
+        static_assert(W % 64 == 0, "16-bit tiles: one butterfly index per wave");
+        if (a.row_exp)
+        {
+            const int jw = __builtin_amdgcn_readfirstlane(jj);
+            const int *rexp = a.row_exp + gb * a.rexp_dist + sub;
+            int ex[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) ex[r] = rexp[(long long)a.P * (jw + r * JPC)];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) v[r] = scal(v[r], (T)ldexpf(1.0f, -ex[r]));
+        }
     }
     // ---- stage 0: radix 16, Ns = 1; outputs at e = 16 jj + q
     dft16(v);

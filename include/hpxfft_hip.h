@@ -166,7 +166,9 @@ int hpxfft_exec_c2r_2d(hpxfft_plan plan, const void *d_spec, void *d_real);
  * 0), "h2d", "d2h"; "plan" (planning seconds) and "plan_flops"; and, when
  * the column pass runs as two launches (Nx > 1024, or > 512 for 16-bit
  * storage), "column_a" and "column_b" (seconds of each launch of the last
- * forward exec; 0 otherwise).  Unknown keys
+ * exec; 0 otherwise).  The backward execs (adjoint, c2r) run the column
+ * pass first: for them "first_fftw" is the column pass and "second_fftw"
+ * the row pass; batched plans give their row and column launches.  Unknown keys
  * give 0.0, as std::map::operator[] does in the reference.  Waits for the
  * last exec to finish. */
 int hpxfft_get_measurement(hpxfft_plan plan, const char *key, double *value);
@@ -191,8 +193,13 @@ int hpxfft_plan_r2c_2d_batched(hpxfft_plan *plan, size_t count, const size_t *nx
                                int effort, void *stream);
 
 /* d_grids: host array of `count` device pointers, one per grid (in place).
- * Waits for the plan's previous batched exec, then uploads the pointers
- * (a host-to-device copy on the plan's stream): not for graph capture. */
+ * When they differ from the previous exec's, their descriptors are uploaded
+ * by a host-to-device copy on the plan's stream (two pinned staging slots
+ * used in turn; the host waits only if the copy made two calls before has
+ * not run yet).  With the same pointers as the previous exec, nothing but
+ * kernels and event records is enqueued, so such a call may be captured
+ * into a HIP graph; capturing a call with a new pointer set gives
+ * HPXFFT_INVALID_ARG. */
 int hpxfft_exec_r2c_2d_batched(hpxfft_plan plan, void *const *d_grids);
 
 /* No reference counterpart: overrides the task shape a batched plan runs in
