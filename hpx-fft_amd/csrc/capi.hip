@@ -233,6 +233,11 @@ struct hpxfft_plan_s
     int device = 0;
     int lgx = 0, lgL = 0;
     size_t ncols = 0;
+    // row kernel for power-of-two rows of 2^13 / 2^14 complex points with FP32
+    // arithmetic: 0 = radix-16 stages (row_r2c_big_kernel), 1 = lane-pair
+    // radix-32 stages (row_r2c_pair_kernel)
+    int row_kind = 0;
+    double row_ms[2] = {};   // measure effort: row pass ms of each kind
     // column strategy
     bool four_step = false;
     int lgP = 0, lgQ = 0;
@@ -673,6 +678,10 @@ std::string describe(const hpxfft_plan_s *p)
            << p->gen_rows << " row(s)/workgroup, "
            << gen_threads(gen_lds_bytes(p, p->grow, (size_t)p->gen_rows * p->grow.n, (size_t)p->gen_rows), p->grow)
            << " threads + r2c split\n";
+    else if (p->row_kind && row_pair_covers(p->prec, p->lgL))
+        os << "  row r2c kernel: " << (size_t(1) << p->lgL) << "-point complex Stockham per row, radix-32 stages "
+           << "split across lane pairs (DPP exchange), radix " << ((size_t(1) << p->lgL) / 1024)
+           << " fused with the r2c split, " << ((size_t(1) << p->lgL) / 32) << " threads\n";
     else
     {
         const geom rg = row_geom(p->prec, p->lgL);
@@ -680,6 +689,9 @@ std::string describe(const hpxfft_plan_s *p)
            << rg.nseq << " row(s)/workgroup, " << rg.nt << " threads, " << rg.E << " elements/thread, radix "
            << (rg.E < 16 ? rg.E : 16) << (rg.split ? ", LDS exchange in re/im halves" : "") << "\n";
     }
+    if (p->row_ms[0] > 0.0)
+        os << "  row kernel autotuned (ms): radix-16 stages " << p->row_ms[0] << ", lane-pair radix-32 stages "
+           << p->row_ms[1] << "\n";
     for (int d = 0; d < 2; ++d)
         if (p->gip_ms[d][0] > 0.0)
             os << "  general " << (d ? "column" : "row") << " pass autotuned (ms): ping-pong " << p->gip_ms[d][0]
@@ -1035,6 +1047,17 @@ int check_aligned(const hpxfft_plan_s *p, const void *a, const void *b)
     return HPXFFT_OK;
 }
 
+// the row pass of power-of-two rows with row kernel `kind` (0: radix-16 stages;
+// 1: lane-pair radix-32 stages, where the plan's rows allow it)
+hipError_t launch_rows_pow2(const hpxfft_plan_s *p, const RowArgs &r, int kind)
+{
+    if (kind && row_pair_covers(p->prec, p->lgL))
+        return row_r2c_pair(p->prec, p->lgL, r, (unsigned)(p->nx * p->batch), p->stream);
+    const geom rg = row_geom(p->prec, p->lgL);
+    const unsigned gx = (unsigned)((p->nx + rg.nseq - 1) / rg.nseq);
+    return row_r2c(p->prec, p->lgL, r, gx, (unsigned)p->batch, p->stream);
+}
+
 int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_exp, int *col_exp)
 {
     if (!d_in || !d_out) return fail(HPXFFT_INVALID_ARG, "null device pointer");
@@ -1086,9 +1109,7 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     }
     else
     {
-        const geom rg = row_geom(p->prec, p->lgL);
-        const unsigned gx = (unsigned)((p->nx + rg.nseq - 1) / rg.nseq);
-        HIPCHK(row_r2c(p->prec, p->lgL, r, gx, (unsigned)p->batch, p->stream));
+        HIPCHK(launch_rows_pow2(p, r, p->row_kind));
     }
     if (p->timed) HIPCHK(hipEventRecord(p->ev[1], p->stream));
     if (int rc = run_columns(p, d_out, d_out, false, s16 ? col_exp : nullptr, p->timed ? p->ev[3] : nullptr))
@@ -1100,6 +1121,10 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     p->h2d = p->d2h = 0.0;
     return HPXFFT_OK;
 }
+
+// timed repetitions per candidate of the planner's searches: measure 3,
+// patient 5, exhaustive 10 (more repetitions, less timing noise in the choice)
+int tune_reps(int effort) { return effort >= HPXFFT_EXHAUSTIVE ? 10 : effort >= HPXFFT_PATIENT ? 5 : 3; }
 
 // four-step splits Nx = P*Q with both sub-lengths in [2^COL_TILE_MIN_LG, 2^COL_TILE_MAX_LG]
 void split_range(int lgx, int &lo, int &hi)
@@ -1133,7 +1158,7 @@ int tune_split(hpxfft_plan_s *p)
     {
         p->lgP = l;
         p->lgQ = p->lgx - l;
-        constexpr int REPS = 3;
+        const int REPS = tune_reps(p->effort);
         rc = run_columns(p, tmp, tmp, false, s16 ? p->col_exp_own : nullptr);   // warm-up
         if (rc == HPXFFT_OK) e = hipEventRecord(p->ev[0], p->stream);
         for (int r = 0; r < REPS && rc == HPXFFT_OK && e == hipSuccess; ++r)
@@ -1184,7 +1209,7 @@ int tune_gen_ip(hpxfft_plan_s *p)
     {
         if (row_ok) p->grow.ip = ip;
         if (col_ok) p->gcol.ip = ip;
-        constexpr int REPS = 3;
+        const int REPS = tune_reps(p->effort);
         double row_s = 0.0, col_s = 0.0;
         for (int r = 0; r <= REPS && rc == HPXFFT_OK && e == hipSuccess; ++r)   // r = 0: warm-up
         {
@@ -1230,7 +1255,7 @@ int tune_gen_columns(hpxfft_plan_s *p)
     for (int two = 0; two < 2 && e == hipSuccess && rc == HPXFFT_OK; ++two)
     {
         p->g2c.on = two == 1;
-        constexpr int REPS = 3;
+        const int REPS = tune_reps(p->effort);
         rc = run_columns(p, tmp, tmp, false, nullptr);   // warm-up
         if (rc == HPXFFT_OK) e = hipEventRecord(p->ev[0], p->stream);
         for (int r = 0; r < REPS && rc == HPXFFT_OK && e == hipSuccess; ++r) rc = run_columns(p, tmp, tmp, false, nullptr);
@@ -1248,6 +1273,62 @@ int tune_gen_columns(hpxfft_plan_s *p)
         p->g2c.avail = false;
     }
     if (rc != HPXFFT_OK) return rc;
+    HIPCHK(e);
+    return HPXFFT_OK;
+}
+
+// patient effort and above: time both row kernels of a plan whose rows take
+// either (2^13 / 2^14 complex points, FP32 arithmetic) on a zeroed grid of the
+// plan's shape and keep the faster (measured: the radix-16 kernel, 15-45%
+// ahead on MI355X).  If the buffers cannot be allocated the default stays.
+int tune_rows(hpxfft_plan_s *p)
+{
+    if (p->gen_row || !row_pair_covers(p->prec, p->lgL)) return HPXFFT_OK;
+    const size_t sb = storage_bytes(p->prec);
+    const size_t bin = p->batch * p->dist_in * sb, bout = p->batch * p->dist_out * 2 * sb;
+    void *tin = nullptr, *tout = nullptr;
+    if (hipMalloc(&tin, bin) != hipSuccess || hipMalloc(&tout, bout) != hipSuccess)
+    {
+        (void)hipGetLastError();
+        (void)hipFree(tin);
+        return HPXFFT_OK;
+    }
+    const bool s16 = p->prec == P_F16 || p->prec == P_BF16;
+    RowArgs r{};
+    r.in = tin;
+    r.out = tout;
+    r.ld_in = (long long)p->ld_in;
+    r.ld_out = (long long)(2 * p->ld_out);
+    r.dist_in = (long long)p->dist_in;
+    r.dist_out = (long long)(2 * p->dist_out);
+    r.nrows = (int)p->nx;
+    r.tw = tw_ptr(p, p->ny);
+    r.tw64 = tw64_ptr(p, p->ny);
+    r.out_exp = s16 ? p->mid_exp : nullptr;
+    r.exp_dist = (long long)p->nx;
+    const int REPS = tune_reps(p->effort);
+    hipError_t e = hipMemsetAsync(tin, 0, bin, p->stream);
+    double best = 0.0;
+    for (int kind = 0; kind < 2 && e == hipSuccess; ++kind)
+    {
+        e = launch_rows_pow2(p, r, kind);   // warm-up
+        if (e == hipSuccess) e = hipEventRecord(p->ev[0], p->stream);
+        for (int i = 0; i < REPS && e == hipSuccess; ++i) e = launch_rows_pow2(p, r, kind);
+        if (e == hipSuccess) e = hipEventRecord(p->ev[2], p->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(p->ev[2]);
+        if (e == hipSuccess)
+        {
+            p->row_ms[kind] = elapsed(p->ev[0], p->ev[2]) * 1e3 / REPS;
+            if (kind == 0 || p->row_ms[kind] < best)
+            {
+                best = p->row_ms[kind];
+                p->row_kind = kind;
+            }
+        }
+    }
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipFree(tin);
+    (void)hipFree(tout);
     HIPCHK(e);
     return HPXFFT_OK;
 }
@@ -1444,6 +1525,8 @@ int plan_common(hpxfft_plan_s *p)
     if (int rc = alloc_events(p)) return rc;
     if (p->four_step && p->effort >= HPXFFT_MEASURE)
         if (int rc = tune_split(p)) return rc;
+    if (p->effort >= HPXFFT_PATIENT)
+        if (int rc = tune_rows(p)) return rc;
     if (p->effort >= HPXFFT_MEASURE && ((p->gen_row && !p->g2r.on && gen_can_ip(p->grow)) ||
                                         (p->gen_col && !p->g2c.on && gen_can_ip(p->gcol))))
         if (int rc = tune_gen_ip(p)) return rc;
@@ -1959,7 +2042,8 @@ int batched_autotune(hpxfft_plan_s *p)
             {
                 const int nb = std::min(p->bblocks[ph][c], std::max(1, p->bntasks[ph][c]));
                 float ms = 0.0f, acc = 0.0f;
-                for (int rep = 0; rep < 5 && !rc; ++rep)
+                const int reps = 2 + tune_reps(p->effort);   // two warm-up runs, then timed
+                for (int rep = 0; rep < reps && !rc; ++rep)
                 {
                     hipLaunchKernelGGL(hpxfft_dev::zero_u32_kernel<0>, dim3(1), dim3(64), 0, p->stream,
                                        p->bticket + ph, 1LL);
@@ -1972,7 +2056,7 @@ int batched_autotune(hpxfft_plan_s *p)
                     (void)hipEventElapsedTime(&ms, a, b);
                     if (rep >= 2) acc += ms;   // two warm-up runs
                 }
-                p->btune_ms[ph][c] = acc / 3.0;
+                p->btune_ms[ph][c] = acc / (reps - 2);
                 if (p->btune_ms[ph][c] < best)
                 {
                     best = p->btune_ms[ph][c];
@@ -2128,6 +2212,18 @@ int hpxfft_plan_set_column_split(hpxfft_plan plan, int log2_p)
     // 16-bit column maxima were sized for the largest P of the range
     plan->lgP = log2_p;
     plan->lgQ = plan->lgx - log2_p;
+    return HPXFFT_OK;
+}
+
+int hpxfft_plan_set_row_kernel(hpxfft_plan plan, int kind)
+{
+    if (!plan || plan->batched) return fail(HPXFFT_INVALID_ARG, "invalid plan");
+    if (kind < 0 || kind > 1) return fail(HPXFFT_INVALID_ARG, "row kernel kind out of range (0..1)");
+    if (kind != 0 && (plan->gen_row || !row_pair_covers(plan->prec, plan->lgL)))
+        return fail(HPXFFT_INVALID_ARG, "the lane-pair row kernel covers rows of 2^13 and 2^14 complex points "
+                                        "(Ny = 16384, 32768) with FP32 arithmetic; this plan has Ny = " +
+                                            std::to_string(plan->ny) + ", precision " + prec_name(plan->prec));
+    plan->row_kind = kind;
     return HPXFFT_OK;
 }
 

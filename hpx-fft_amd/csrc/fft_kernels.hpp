@@ -121,8 +121,9 @@ __device__ __forceinline__ void row_r2c_body(const RowArgs &a, long long b, int 
     SO *out = reinterpret_cast<SO *>(a.out) + b * a.dist_out;
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(a.tw);
 
-    // loads from in-bounds rows only, unguarded (rows past the end read the
-    // last row and take 0), so all E are in flight before the first wait
+    // 16-bit storage: unguarded loads from in-bounds rows (rows past the end
+    // read the last row and take 0), so all E are in flight before the first
+    // wait; wider storage keeps the guarded loads (as col_body)
     cpx<T> v[E];
 #pragma unroll
     for (int x = 0; x < E; ++x)
@@ -130,8 +131,14 @@ __device__ __forceinline__ void row_r2c_body(const RowArgs &a, long long b, int 
         int seq, e;
         EN::in_elem(t, x, seq, e);
         const int row = row0 + seq, rowc = row < a.nrows ? row : a.nrows - 1;
-        cpx<T> z = stor<SI>::ld(in + (long long)rowc * a.ld_in + 2 * e);
-        if (row >= a.nrows) z = cpx<T>{T(0), T(0)};
+        cpx<T> z = {T(0), T(0)};
+        if constexpr (is16<SI>())
+        {
+            z = stor<SI>::ld(in + (long long)rowc * a.ld_in + 2 * e);
+            if (row >= a.nrows) z = cpx<T>{T(0), T(0)};
+        }
+        else if (row < a.nrows)
+            z = stor<SI>::ld(in + (long long)row * a.ld_in + 2 * e);
         v[x] = z;
     }
     if (a.in_exp)
@@ -290,17 +297,27 @@ __device__ __forceinline__ void col_body(const ColArgs &a, long long b, int stri
     const SI *in = reinterpret_cast<const SI *>(a.in) + 2 * b * a.dist;   // dist counts complex elements
     SO *out = reinterpret_cast<SO *>(a.out) + 2 * b * a.dist;
 
-    // unguarded loads from in-bounds columns (lanes past the last column read
-    // it and take 0): all E in flight before the first wait
+    // 16-bit storage: unguarded loads from in-bounds columns (lanes past the
+    // last column read it and take 0), so all E are in flight before the first
+    // wait (a guarded load followed by its conversion cost one round trip
+    // each).  Wider storage keeps the guarded loads, which the compiler
+    // already issues together and which hold fewer registers in the
+    // persistent batched kernel built on this body.
     cpx<T> v[E];
 #pragma unroll
     for (int x = 0; x < E; ++x)
     {
         int w, e;
         EN::in_elem(t, x, w, e);
-        const int wc = w < nvalid ? w : nvalid - 1;
-        cpx<T> z = stor<SI>::ld(in + 2 * ((long long)e * a.pitch + col0 + wc));
-        if (w >= nvalid) z = cpx<T>{T(0), T(0)};
+        cpx<T> z = {T(0), T(0)};
+        if constexpr (is16<SI>())
+        {
+            const int wc = w < nvalid ? w : nvalid - 1;
+            z = stor<SI>::ld(in + 2 * ((long long)e * a.pitch + col0 + wc));
+            if (w >= nvalid) z = cpx<T>{T(0), T(0)};
+        }
+        else if (w < nvalid)
+            z = stor<SI>::ld(in + 2 * ((long long)e * a.pitch + col0 + w));
         if constexpr (CONJ) z = conjc(z);
         v[x] = z;
     }

@@ -3,6 +3,7 @@
 
 #include "fft_kernels.hpp"
 #include "fft_row_big.hpp"
+#include "fft_row_pair.hpp"
 #include "fft_tile.hpp"
 #include <algorithm>
 
@@ -28,6 +29,7 @@ struct geom
 // per precision (defined in inst_<prec>.hip)
 #define HPXFFT_DECL_LAUNCH(sfx)                                                                                    \
     hipError_t row_r2c_##sfx(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s);                   \
+    hipError_t row_r2c_pair_##sfx(int lg, const RowArgs &a, unsigned total, hipStream_t s);                       \
     hipError_t row_c2r_##sfx(int lg, bool halve, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s);       \
     hipError_t col_##sfx(int lg, int mode, bool conj, const ColArgs &a, unsigned gx, unsigned gy, unsigned gz,    \
                          hipStream_t s);                                                                           \
@@ -40,6 +42,20 @@ HPXFFT_DECL_LAUNCH(f64)
 HPXFFT_DECL_LAUNCH(f32)
 HPXFFT_DECL_LAUNCH(f16)
 HPXFFT_DECL_LAUNCH(bf16)
+
+// rows of 2^13 / 2^14 complex points with lane-pair radix-32 stages
+// (fft_row_pair.hpp; FP32 arithmetic: FP32 and 16-bit storage); `total` rows
+inline hipError_t row_r2c_pair(int p, int lg, const RowArgs &a, unsigned total, hipStream_t s)
+{
+    switch (p)
+    {
+    case P_F32: return row_r2c_pair_f32(lg, a, total, s);
+    case P_F16: return row_r2c_pair_f16(lg, a, total, s);
+    case P_BF16: return row_r2c_pair_bf16(lg, a, total, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+inline bool row_pair_covers(int prec, int lg) { return prec != P_F64 && (lg == 13 || lg == 14); }
 
 inline hipError_t row_r2c(int p, int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
 {
@@ -140,6 +156,16 @@ hipError_t launch_row_r2c(const RowArgs &a, unsigned gx, unsigned gy, hipStream_
     {
         hipLaunchKernelGGL((row_r2c_kernel<T, S, S, LG>), dim3(gx, gy), dim3(cfg<T, S>::row_NT(LG)), 0, s, a);
     }
+    return hipGetLastError();
+}
+
+template <typename T, typename S>
+hipError_t launch_row_r2c_pair(int lg, const RowArgs &a, unsigned total, hipStream_t s)
+{
+    if constexpr (sizeof(T) == 8) return hipErrorInvalidValue;   // FP64 rows keep the radix-16 kernel
+    else if (lg == 13) hipLaunchKernelGGL((row_r2c_pair_kernel<T, S, S, 13>), dim3(total), dim3(256), 0, s, a);
+    else if (lg == 14) hipLaunchKernelGGL((row_r2c_pair_kernel<T, S, S, 14>), dim3(total), dim3(512), 0, s, a);
+    else return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
@@ -250,6 +276,10 @@ template <typename T, typename S> geom tile_geom_t(int lg)
     hipError_t row_r2c_##sfx(int lg, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)                    \
     {                                                                                                              \
         return detail::row_r2c_sw<T, S>(lg, a, gx, gy, s);                                                         \
+    }                                                                                                              \
+    hipError_t row_r2c_pair_##sfx(int lg, const RowArgs &a, unsigned total, hipStream_t s)                        \
+    {                                                                                                              \
+        return detail::launch_row_r2c_pair<T, S>(lg, a, total, s);                                                 \
     }                                                                                                              \
     hipError_t row_c2r_##sfx(int lg, bool halve, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)        \
     {                                                                                                              \

@@ -29,7 +29,7 @@ EXPORTS = (
     "hpxfft_exec_r2c_2d_scaled", "hpxfft_exec_r2c_2d_host", "hpxfft_exec_r2c_2d_adjoint", "hpxfft_exec_c2r_2d", "hpxfft_get_measurement",
     "hpxfft_write_plan", "hpxfft_plan_describe", "hpxfft_plan_destroy", "hpxfft_plan_r2c_2d_batched",
     "hpxfft_exec_r2c_2d_batched", "hpxfft_plan_batched_set_config", "hpxfft_plan_set_column_split",
-    "hpxfft_exec_r2c_2d_noalloc", "hpxfft_last_error", "hpxfft_version",
+    "hpxfft_exec_r2c_2d_noalloc", "hpxfft_plan_set_row_kernel", "hpxfft_last_error", "hpxfft_version",
 )
 
 
@@ -69,6 +69,7 @@ def lib() -> ctypes.CDLL:
     L.hpxfft_plan_batched_set_config.argtypes = [vp, ci, ci]
     L.hpxfft_plan_set_column_split.argtypes = [vp, ci]
     L.hpxfft_exec_r2c_2d_noalloc.argtypes = [sz, sz, sz, sz, ci, vp, vp, vp]
+    L.hpxfft_plan_set_row_kernel.argtypes = [vp, ci]
     L.hpxfft_last_error.argtypes = []
     L.hpxfft_last_error.restype = ctypes.c_char_p
     L.hpxfft_version.argtypes = []

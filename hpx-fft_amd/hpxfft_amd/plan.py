@@ -84,6 +84,11 @@ class Plan:
         use it to check every split the measure effort may keep)."""
         check(lib().hpxfft_plan_set_column_split(self._h, int(log2_p)))
 
+    def set_row_kernel(self, kind: int) -> None:
+        """Rows of 2^13 / 2^14 complex points, FP32 arithmetic: 0 = radix-16
+        stages, 1 = radix-32 stages across lane pairs."""
+        check(lib().hpxfft_plan_set_row_kernel(self._h, int(kind)))
+
     def measurement(self, key: str) -> float:
         v = ctypes.c_double(0.0)
         check(lib().hpxfft_get_measurement(self._h, key.encode(), ctypes.byref(v)))

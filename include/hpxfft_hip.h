@@ -61,13 +61,16 @@ typedef enum
     HPXFFT_BF16_STORAGE = 3  /* bfloat16 storage, FP32 arithmetic, same scaling */
 } hpxfft_precision;
 
-/* Planning effort, the image of the FFTW plan-flag strings. */
+/* Planning effort, the image of the FFTW plan-flag strings (wider searches,
+ * as FFTW_MEASURE < FFTW_PATIENT < FFTW_EXHAUSTIVE, adapter_fftw.hpp:13-18). */
 typedef enum
 {
-    HPXFFT_ESTIMATE = 0,
-    HPXFFT_MEASURE = 1,    /* time the candidate kernel configurations and keep the fastest */
-    HPXFFT_PATIENT = 2,
-    HPXFFT_EXHAUSTIVE = 3
+    HPXFFT_ESTIMATE = 0,   /* no timing: balanced four-step split, radix-16 row kernel, in-place general stages */
+    HPXFFT_MEASURE = 1,    /* time the four-step splits, the general kernels' stage forms and one/two levels,
+                              the batched task shapes; keep the fastest (3 timed runs per candidate) */
+    HPXFFT_PATIENT = 2,    /* measure's search, plus the row kernels of 2^13- / 2^14-point rows (FP32
+                              arithmetic); 5 timed runs per candidate */
+    HPXFFT_EXHAUSTIVE = 3  /* patient's search with 10 timed runs per candidate */
 } hpxfft_effort;
 
 typedef struct hpxfft_plan_s *hpxfft_plan;
@@ -217,6 +220,15 @@ int hpxfft_plan_batched_set_config(hpxfft_plan plan, int row_cfg, int col_cfg);
  * for a plan whose column pass is not a four-step (the plan text says which
  * it is).  Lets a test check every split the measure effort may keep. */
 int hpxfft_plan_set_column_split(hpxfft_plan plan, int log2_p);
+
+/* No reference counterpart: selects the row kernel of a plan whose rows are
+ * 2^13 or 2^14 complex points (Ny = 16384 or 32768, powers of two) with FP32
+ * arithmetic (FP32, FP16 or BF16 storage): 0 = radix-16 Stockham stages,
+ * three LDS exchanges per row; 1 = radix-32 stages whose butterflies span a
+ * lane pair (the pair exchanges values by DPP) and a radix-8/16 stage fused
+ * with the r2c split, two LDS exchanges per row.  HPXFFT_INVALID_ARG for
+ * another kind, or kind 1 on a plan with other rows or FP64 arithmetic. */
+int hpxfft_plan_set_row_kernel(hpxfft_plan plan, int kind);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *hpxfft_last_error(void);
