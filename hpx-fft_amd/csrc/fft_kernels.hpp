@@ -421,6 +421,22 @@ row_c2r_kernel(RowArgs a)
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(a.tw);
     const int row0 = bk.x * NSEQ;
 
+    // loads first, unguarded: tasks past the rows read the last row (an
+    // in-bounds address) and take 0, so the 2 NTASK data loads and NTASK
+    // twiddle loads are all in flight before the first wait (a guarded load
+    // in a divergent branch is waited on inside that branch)
+    cpx<T> G0[NTASK], G1[NTASK], W[NTASK];
+#pragma unroll
+    for (int i = 0; i < NTASK; ++i)
+    {
+        const int tau = t + i * NT;
+        const int seq = tau / HK, k = tau - seq * HK;
+        const int row = row0 + seq, rowc = row < a.nrows ? row : a.nrows - 1;
+        const SI *irow = in + (long long)rowc * a.ld_in;
+        G0[i] = stor<SI>::ld(irow + 2 * k);
+        G1[i] = stor<SI>::ld(irow + 2 * (L - k));
+        W[i] = tw[k];
+    }
     // pre-processing: pairs (k, L-k) -> conj Z[k], conj Z[L-k]
     cpx<T> Z0[NTASK], Z1[NTASK];
 #pragma unroll
@@ -429,13 +445,9 @@ row_c2r_kernel(RowArgs a)
         const int tau = t + i * NT;
         const int seq = tau / HK, k = tau - seq * HK;
         const int row = row0 + seq;
-        cpx<T> g0 = {T(0), T(0)}, g1 = {T(0), T(0)};
-        if (tau < NSEQ * HK && row < a.nrows)
-        {
-            const SI *irow = in + (long long)row * a.ld_in;
-            g0 = stor<SI>::ld(irow + 2 * k);
-            g1 = stor<SI>::ld(irow + 2 * (L - k));
-        }
+        const bool live = tau < NSEQ * HK && row < a.nrows;
+        const cpx<T> g0 = live ? G0[i] : cpx<T>{T(0), T(0)};
+        const cpx<T> g1 = live ? G1[i] : cpx<T>{T(0), T(0)};
         if (k == 0)
         {
             // H0 = Re G0, HL = Re GL: Z0 = (H0 + HL) + i (H0 - HL)
@@ -446,7 +458,7 @@ row_c2r_kernel(RowArgs a)
         {
             const T hs = HALVE ? T(0.5) : T(1);
             const cpx<T> h0 = scal(g0, hs), h1 = scal(g1, hs);
-            const cpx<T> w = tw[k];
+            const cpx<T> w = W[i];
             const cpx<T> wc = conjc(w);
             // Z[k]
             const cpx<T> s0 = h0 + conjc(h1), d0 = cmul(h0 - conjc(h1), wc);
