@@ -221,7 +221,15 @@ struct gen2_dim
     int P = 0, Q = 0;
     hpxfft_dev::GenPlan A{}, B{};
     int WA = 1, WB = 1;                    // sequences per workgroup in each phase
-    size_t tabA = 0, tabB = 0, tabT = 0;   // table offsets (complex entries): plans A, B and w_N^m
+    size_t tabA = 0, tabB = 0, tabT = 0;   // table offsets (complex entries): plans A, B and w_{PQ}^m
+    int tw_shift = 0;                      // w_{PQ} table split at 2^tw_shift (0: one table)
+    // Bluestein over two levels: the length-N DFT as a chirp convolution of
+    // length M = P*Q (a power of two >= 2N - 1), three passes
+    bool blue = false;
+    size_t N = 0;
+    size_t tabC = 0, tabBh = 0;            // the chirp table w_{2N} and Bh = DFT_M(b) / M
+    int chirp_shift = 0;
+    size_t points() const { return (size_t)P * (size_t)Q; }   // scratch points per sequence
 };
 
 struct hpxfft_plan_s
@@ -255,6 +263,7 @@ struct hpxfft_plan_s
     double gip_ms[2][2] = {};      // measure effort: general [row, column] pass ms, [ping-pong, in place]
     double *gen_tab = nullptr;     // exact FP64 tables w_n^m: rows L = Ny/2, split Ny, columns Nx
     size_t gtab_L = 0, gtab_2 = 0, gtab_N = 0;   // their offsets (complex entries)
+    int tw2_shift = 0;                            // the w_Ny table split at 2^tw2_shift (0: whole)
     // device memory
     dev_tables tabs;
     void *scratch = nullptr;
@@ -349,7 +358,8 @@ int gen_allow_lds(int dev)
         GEN_BOTH(gen_row_c2r_kernel, double, false), GEN_BOTH(gen_row_c2r_kernel, double, true),
         GEN_BOTH(gen_row_c2r_kernel, float, false), GEN_BOTH(gen_row_c2r_kernel, float, true),
         GEN_BOTH(gen_strided_kernel, double), GEN_BOTH(gen_strided_kernel, float),
-        GEN_BOTH(gen_rows_split_kernel, double), GEN_BOTH(gen_rows_split_kernel, float)};
+        GEN_BOTH(gen_rows_split_kernel, double), GEN_BOTH(gen_rows_split_kernel, float),
+        GEN_BOTH(gen_blue_mid_kernel, double), GEN_BOTH(gen_blue_mid_kernel, float)};
 #undef GEN_BOTH
     // the buffers plus up to 1024 words of per-row / per-column maxima
     for (const void *f : fns)
@@ -465,11 +475,29 @@ bool gen_setup_in_place_only(size_t n, size_t cap, hpxfft_dev::GenPlan &g)
 }
 
 
-// two-level split N = P * Q (N <= 2^20): both sub-lengths within one
+// tables of more than 2^20 entries are split in two of about sqrt(n) entries
+constexpr size_t GEN_FULL_TABLE = size_t(1) << 20;
+int split_shift(size_t n)
+{
+    if (n <= GEN_FULL_TABLE) return 0;
+    int s = 0;
+    while ((size_t(1) << (2 * s)) < n) ++s;
+    return s;
+}
+// complex entries of w_n^m (m < n), whole or split at 2^s
+size_t tw_table_size(size_t n, int s) { return s ? (size_t(1) << s) + ((n + (size_t(1) << s) - 1) >> s) : n; }
+
+// longest two-level length, and longest length Bluestein's algorithm takes
+// over two levels (its convolution length, a power of two < 2^24, is the
+// scratch per sequence)
+constexpr size_t GEN2_MAX = size_t(1) << 24;
+constexpr size_t GEN2_BLUE_MAX = size_t(1) << 22;
+
+// two-level split N = P * Q (N <= 2^24): both sub-lengths within one
 // workgroup, direct plans preferred over Bluestein, then the most balanced
 bool gen2_setup(size_t N, size_t cap, gen2_dim &d)
 {
-    if (N > (size_t(1) << 20)) return false;
+    if (N > GEN2_MAX) return false;
     int best = -1;
     double best_score = 0.0;
     for (size_t P = 2; P <= cap && P <= N / 2; ++P)
@@ -496,6 +524,33 @@ bool gen2_setup(size_t N, size_t cap, gen2_dim &d)
     d.on = d.avail = true;
     d.P = best;
     d.Q = (int)(N / (size_t)best);
+    d.tw_shift = split_shift(N);
+    d.blue = false;
+    return true;
+}
+
+// Bluestein's algorithm over two levels for a length N <= 2^22 that neither
+// one workgroup nor a split into two one-workgroup lengths covers: the
+// convolution length M (the power of two >= 2N - 1) as M = P*Q, P <= Q,
+// both direct power-of-two plans (ping-pong stages)
+bool gen2b_setup(size_t N, size_t cap, gen2_dim &d)
+{
+    if (N < 2 || N > GEN2_BLUE_MAX) return false;
+    size_t M = 1;
+    while (M < 2 * N - 1) M *= 2;
+    size_t P = 1;
+    while (P * P < M) P *= 2;
+    if (P * P > M) P /= 2;
+    const size_t Q = M / P;
+    if (Q > cap || !gen_setup(Q, cap, d.A) || !gen_setup(P, cap, d.B)) return false;
+    d.A.ip = d.B.ip = 0;
+    d.on = d.avail = true;
+    d.P = (int)P;
+    d.Q = (int)Q;
+    d.tw_shift = split_shift(M);
+    d.blue = true;
+    d.N = N;
+    d.chirp_shift = split_shift(2 * N);
     return true;
 }
 
@@ -532,39 +587,23 @@ int gen_digit_rev(const hpxfft_dev::GenPlan &g, int e)
     return pos;
 }
 
-// fill a general plan's table at h[2*off ...] (complex FP64 pairs), from long
-// double values: w_n^m exact (octant-reduced), the chirp c[j] = w_{2len}^{j^2
-// mod 2len} exact, and Bh = DFT_n(b) / n with b[m] = conj c[m], b[n-m] = conj
-// c[m] (0 < m < len), by a long-double radix-2 FFT
-void gen_table(const hpxfft_dev::GenPlan &g, std::vector<double> &h, size_t off)
+// the chirp c[j] = w_{2len}^{j^2 mod 2len} (j < len) and DFT_n(b) with
+// b[m] = conj c[m], b[n-m] = conj c[m] (0 < m < len), in long double (n a
+// power of two >= 2 len - 1; an iterative radix-2 FFT)
+void blue_spectrum(uint64_t len, uint64_t n, std::vector<long double> &cr, std::vector<long double> &ci,
+                   std::vector<long double> &br, std::vector<long double> &bi)
 {
-    const uint64_t n = (uint64_t)g.n;
-    for (uint64_t m = 0; m < n; ++m)
-    {
-        long double c, sn;
-        twiddle_ld(m, n, c, sn);
-        h[2 * (off + m)] = (double)c;
-        h[2 * (off + m) + 1] = -(double)sn;
-    }
-    if (g.off_r >= 0)
-    {
-        std::vector<int> pos(((size_t)n + 3) / 4 * 4, 0);
-        for (uint64_t e = 0; e < n; ++e) pos[e] = gen_digit_rev(g, (int)e);
-        std::memcpy(&h[2 * (off + (size_t)g.off_r)], pos.data(), sizeof(int) * pos.size());
-    }
-    if (g.len == g.n) return;
-    const uint64_t len = (uint64_t)g.len;
-    std::vector<long double> cr(len), ci(len);
+    cr.assign(len, 0.0L);
+    ci.assign(len, 0.0L);
     for (uint64_t j = 0; j < len; ++j)
     {
         long double c, sn;
         twiddle_ld((j * j) % (2 * len), 2 * len, c, sn);
         cr[j] = c;
         ci[j] = -sn;
-        h[2 * (off + n + j)] = (double)c;
-        h[2 * (off + n + j) + 1] = -(double)sn;
     }
-    std::vector<long double> br(n, 0.0L), bi(n, 0.0L);
+    br.assign(n, 0.0L);
+    bi.assign(n, 0.0L);
     for (uint64_t m = 0; m < len; ++m)
     {
         br[m] = cr[m];
@@ -604,6 +643,36 @@ void gen_table(const hpxfft_dev::GenPlan &g, std::vector<double> &h, size_t off)
                 bi[a] += ti;
             }
         }
+    }
+}
+
+// fill a general plan's table at h[2*off ...] (complex FP64 pairs), from long
+// double values: w_n^m exact (octant-reduced), the chirp c[j] = w_{2len}^{j^2
+// mod 2len} exact, and Bh = DFT_n(b) / n (blue_spectrum)
+void gen_table(const hpxfft_dev::GenPlan &g, std::vector<double> &h, size_t off)
+{
+    const uint64_t n = (uint64_t)g.n;
+    for (uint64_t m = 0; m < n; ++m)
+    {
+        long double c, sn;
+        twiddle_ld(m, n, c, sn);
+        h[2 * (off + m)] = (double)c;
+        h[2 * (off + m) + 1] = -(double)sn;
+    }
+    if (g.off_r >= 0)
+    {
+        std::vector<int> pos(((size_t)n + 3) / 4 * 4, 0);
+        for (uint64_t e = 0; e < n; ++e) pos[e] = gen_digit_rev(g, (int)e);
+        std::memcpy(&h[2 * (off + (size_t)g.off_r)], pos.data(), sizeof(int) * pos.size());
+    }
+    if (g.len == g.n) return;
+    const uint64_t len = (uint64_t)g.len;
+    std::vector<long double> cr, ci, br, bi;
+    blue_spectrum(len, n, cr, ci, br, bi);
+    for (uint64_t j = 0; j < len; ++j)
+    {
+        h[2 * (off + n + j)] = (double)cr[j];
+        h[2 * (off + n + j) + 1] = (double)ci[j];
     }
     for (uint64_t m = 0; m < n; ++m)
     {
@@ -658,16 +727,29 @@ std::string describe(const hpxfft_plan_s *p)
     os << "hpxfft-hip plan (gfx950): Nx=" << p->nx << " Ny=" << p->ny << " batch=" << p->batch << " precision "
        << prec_name(p->prec) << " effort " << p->effort << "\n";
     auto two_level = [&](const char *what, const gen2_dim &d) {
+        if (d.blue)
+        {
+            os << "  " << what << " (general length " << d.N << ", Bluestein over two levels): chirp convolution of "
+               << "length M = P*Q = " << d.P << " * " << d.Q << ", three passes through " << d.points()
+               << " scratch points per sequence\n    pass 1: x c, " << d.Q << "-point " << gen_method(d.A)
+               << " over elements m1+P*e, twiddle w_M^(m1 k2), " << d.WA << " sequence(s)/workgroup\n    pass 2: "
+               << d.P << "-point " << gen_method(d.B) << ", times DFT_M(b)/M, inverse, twiddle w_M^(-n1 k2), " << d.WB
+               << " sequence(s)/workgroup\n    pass 3: inverse " << d.Q << "-point, times c, indices n1+P*n2 < "
+               << d.N << "\n" << (d.tw_shift ? "    twiddle tables split in two (FP64 product)\n" : "");
+            return;
+        }
         os << "  " << what << " (general length, two levels): " << (size_t)d.P * d.Q << " = P*Q = " << d.P << " * "
            << d.Q << "\n    phase A: " << d.Q << "-point " << gen_method(d.A) << " over elements m1+P*e, twiddle w^(m1 k2), "
            << d.WA << " sequence(s)/workgroup\n    phase B: " << d.P << "-point " << gen_method(d.B) << ", "
-           << d.WB << " sequence(s)/workgroup\n";
+           << d.WB << " sequence(s)/workgroup\n" << (d.tw_shift ? "    twiddle table split in two (FP64 product)\n" : "");
     };
     if (p->g2r.on)
     {
         two_level("row r2c", p->g2r);
         hpxfft_dev::GenPlan gB;
-        if (const int Wh = rows_split_width(p, gB))
+        if (p->g2r.blue)
+            os << "    then the r2c split in place (one thread per pair k, Ny/2-k)\n";
+        else if (const int Wh = rows_split_width(p, gB))
             os << "    forward: phase B with the r2c split fused in, " << Wh << " columns k2 <= Q/2 and their "
                << "partners Q-k2 per workgroup, " << gen_method(gB) << "\n";
         else
@@ -784,10 +866,185 @@ int launch_strided(hpxfft_plan_s *p, const hpxfft_dev::GenStride &a, const hpxff
 
 unsigned cdiv(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// Bluestein pass 2 of a two-level dimension (gen_blue_mid_kernel), in place in `a`
+int launch_blue_mid(hpxfft_plan_s *p, hpxfft_dev::GenStride a, const gen2_dim &d, int W, unsigned gx, unsigned gy,
+                    unsigned gz)
+{
+    const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
+    const size_t lds = gen_lds_bytes(p, d.B, (size_t)W * d.B.n);
+    const dim3 grid(gx, gy, gz);
+    const unsigned nt = gen_threads(lds, d.B);
+    a.tw_shift = d.tw_shift;
+    if (p->prec == P_F64)
+        GEN_LAUNCH(hpxfft_dev::gen_blue_mid_kernel, (double), nt, grid, lds, p->stream, a, d.B, tab + d.tabB,
+                   tab + d.tabBh, tab + d.tabT, W, d.Q);
+    else
+        GEN_LAUNCH(hpxfft_dev::gen_blue_mid_kernel, (float), nt, grid, lds, p->stream, a, d.B, tab + d.tabB,
+                   tab + d.tabBh, tab + d.tabT, W, d.Q);
+    HIPCHK(hipGetLastError());
+    return HPXFFT_OK;
+}
+
+// the Bluestein fields of a pass-1 / pass-3 GenStride
+void blue_fields(const hpxfft_plan_s *p, const gen2_dim &d, hpxfft_dev::GenStride &a)
+{
+    a.blue_n = (int)d.N;
+    a.blue_p = d.P;
+    a.chirp_shift = d.chirp_shift;
+    a.chirp = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab) + d.tabC;
+}
+
+// two-level Bluestein column pass (Nx = N, M = P*Q) from spec_in to spec_out
+// through the plan's scratch [m1][k2][col] of M rows; conj: the inverse
+int run_columns_blue(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj)
+{
+    const gen2_dim &d = p->g2c;
+    const long long P = d.P, Q = d.Q, M = (long long)d.points(), nc = (long long)p->ncols, ld = (long long)p->ld_out;
+    const unsigned bz = (unsigned)p->batch;
+    hpxfft_dev::GenStride a{};
+    a.in = spec_in;
+    a.in_seq = P * ld;
+    a.in_col = 1;
+    a.in_sub = ld;
+    a.in_bat = (long long)p->dist_out;
+    a.out = p->gscr;
+    a.out_seq = nc;
+    a.out_col = 1;
+    a.out_sub = Q * nc;
+    a.out_bat = M * nc;
+    a.ncols = (int)nc;
+    a.tw_n = (int)M;
+    a.tw_shift = d.tw_shift;
+    a.conj_in = conj;
+    a.blue_in = 1;
+    blue_fields(p, d, a);
+    if (int rc = launch_strided(p, a, d.A, d.tabA, d.tabT, d.WA, cdiv(nc, d.WA), (unsigned)P, bz)) return rc;
+    hpxfft_dev::GenStride m{};
+    m.in = p->gscr;
+    m.out = p->gscr;
+    m.in_seq = m.out_seq = Q * nc;
+    m.in_col = m.out_col = 1;
+    m.in_sub = m.out_sub = nc;
+    m.in_bat = m.out_bat = M * nc;
+    m.ncols = (int)nc;
+    if (int rc = launch_blue_mid(p, m, d, d.WB, cdiv(nc, d.WB), (unsigned)Q, bz)) return rc;
+    hpxfft_dev::GenStride b{};
+    b.in = p->gscr;
+    b.in_seq = nc;
+    b.in_col = 1;
+    b.in_sub = Q * nc;
+    b.in_bat = M * nc;
+    b.out = spec_out;
+    b.out_seq = P * ld;
+    b.out_col = 1;
+    b.out_sub = ld;
+    b.out_bat = (long long)p->dist_out;
+    b.ncols = (int)nc;
+    b.conj_in = b.conj_out = 1;
+    b.blue_out = 1;
+    b.blue_conj = conj;
+    blue_fields(p, d, b);
+    return launch_strided(p, b, d.A, d.tabA, d.tabT, d.WA, cdiv(nc, d.WA), (unsigned)P, bz);
+}
+
+// two-level Bluestein rows (Ny/2 = N, M = P*Q) through the plan's scratch
+// [row][m1][k2] of M points per row.  Forward: the real rows as complex
+// points -> the spectrum rows, then the r2c split in place.  Backward (c2r,
+// halve for the adjoint): the c2r sequence of each half-spectrum row -> `real`.
+int run_rows_blue(hpxfft_plan_s *p, const void *real_in, void *real_out, const void *spec_in, void *spec_out,
+                  bool backward, bool halve)
+{
+    const gen2_dim &d = p->g2r;
+    const long long P = d.P, Q = d.Q, M = (long long)d.points(), L = (long long)d.N, nx = (long long)p->nx;
+    const unsigned by = (unsigned)p->nx, bz = (unsigned)p->batch;
+    hpxfft_dev::GenStride a{};
+    a.in_seq = P;
+    a.in_col = 1;
+    if (!backward)
+    {
+        a.in = real_in;
+        a.in_sub = (long long)p->ld_in / 2;
+        a.in_bat = (long long)p->dist_in / 2;
+    }
+    else
+    {
+        a.in = spec_in;
+        a.in_sub = (long long)p->ld_out;
+        a.in_bat = (long long)p->dist_out;
+        a.pre = halve ? 2 : 1;
+        a.prel = (int)L;
+        a.conj_in = 1;
+        a.tw2_shift = p->tw2_shift;
+    }
+    a.out = p->gtmp;
+    a.out_seq = 1;
+    a.out_col = Q;
+    a.out_sub = M;
+    a.out_bat = nx * M;
+    a.ncols = (int)P;
+    a.tw_n = (int)M;
+    a.tw_col = 1;
+    a.tw_shift = d.tw_shift;
+    a.blue_in = 1;
+    blue_fields(p, d, a);
+    if (int rc = launch_strided(p, a, d.A, d.tabA, d.tabT, d.WA, cdiv(P, d.WA), by, bz)) return rc;
+    hpxfft_dev::GenStride m{};
+    m.in = p->gtmp;
+    m.out = p->gtmp;
+    m.in_seq = m.out_seq = Q;
+    m.in_col = m.out_col = 1;
+    m.in_sub = m.out_sub = M;
+    m.in_bat = m.out_bat = nx * M;
+    m.ncols = (int)Q;
+    m.tw_col = 1;
+    if (int rc = launch_blue_mid(p, m, d, d.WB, cdiv(Q, d.WB), by, bz)) return rc;
+    hpxfft_dev::GenStride b{};
+    b.in = p->gtmp;
+    b.in_seq = 1;
+    b.in_col = Q;
+    b.in_sub = M;
+    b.in_bat = nx * M;
+    b.out_seq = P;
+    b.out_col = 1;
+    b.ncols = (int)P;
+    b.tw_col = 1;
+    b.conj_in = b.conj_out = 1;
+    b.blue_out = 1;
+    blue_fields(p, d, b);
+    if (!backward)
+    {
+        b.out = spec_out;
+        b.out_sub = (long long)p->ld_out;
+        b.out_bat = (long long)p->dist_out;
+    }
+    else
+    {
+        b.out = real_out;
+        b.out_sub = (long long)p->ld_in / 2;
+        b.out_bat = (long long)p->dist_in / 2;
+        b.blue_conj = 1;
+    }
+    if (int rc = launch_strided(p, b, d.A, d.tabA, d.tabT, d.WA, cdiv(P, d.WA), by, bz)) return rc;
+    if (!backward)
+    {
+        const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
+        const dim3 grid(cdiv((size_t)L / 2 + 1, 256), by, bz);
+        if (p->prec == P_F64)
+            hipLaunchKernelGGL(hpxfft_dev::gen_split_kernel<double>, grid, dim3(256), 0, p->stream, spec_out,
+                               (long long)p->ld_out, (long long)p->dist_out, (int)L, tab + p->gtab_2, p->tw2_shift);
+        else
+            hipLaunchKernelGGL(hpxfft_dev::gen_split_kernel<float>, grid, dim3(256), 0, p->stream, spec_out,
+                               (long long)p->ld_out, (long long)p->dist_out, (int)L, tab + p->gtab_2, p->tw2_shift);
+        HIPCHK(hipGetLastError());
+    }
+    return HPXFFT_OK;
+}
+
 // two-level column pass (Nx = P*Q) from spec_in to spec_out through the
 // plan's scratch [m1][k2][col]; conj: the inverse transform
 int run_columns_2level(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bool conj)
 {
+    if (p->g2c.blue) return run_columns_blue(p, spec_in, spec_out, conj);
     const gen2_dim &d = p->g2c;
     const long long P = d.P, Q = d.Q, nc = (long long)p->ncols, ld = (long long)p->ld_out;
     hpxfft_dev::GenStride a{};
@@ -803,6 +1060,7 @@ int run_columns_2level(hpxfft_plan_s *p, const void *spec_in, void *spec_out, bo
     a.out_bat = (long long)p->nx * nc;
     a.ncols = (int)nc;
     a.tw_n = (int)p->nx;
+    a.tw_shift = d.tw_shift;
     a.conj_in = conj;
     if (int rc = launch_strided(p, a, d.A, d.tabA, d.tabT, d.WA, cdiv(nc, d.WA), (unsigned)P, (unsigned)p->batch))
         return rc;
@@ -843,6 +1101,7 @@ int rows_split_width(const hpxfft_plan_s *p, hpxfft_dev::GenPlan &gB)
 int run_rows_2level(hpxfft_plan_s *p, const void *real_in, void *real_out, const void *spec_in, void *spec_out,
                     bool backward, bool halve)
 {
+    if (p->g2r.blue) return run_rows_blue(p, real_in, real_out, spec_in, spec_out, backward, halve);
     const gen2_dim &d = p->g2r;
     const long long P = d.P, Q = d.Q, L = P * Q;
     hpxfft_dev::GenStride a{};
@@ -862,6 +1121,7 @@ int run_rows_2level(hpxfft_plan_s *p, const void *real_in, void *real_out, const
         a.pre = halve ? 2 : 1;
         a.prel = (int)L;
         a.conj_in = 1;
+        a.tw2_shift = p->tw2_shift;
     }
     a.out = p->gtmp;
     a.out_seq = 1;
@@ -871,6 +1131,7 @@ int run_rows_2level(hpxfft_plan_s *p, const void *real_in, void *real_out, const
     a.ncols = (int)P;
     a.tw_n = (int)L;
     a.tw_col = 1;
+    a.tw_shift = d.tw_shift;
     if (int rc = launch_strided(p, a, d.A, d.tabA, d.tabT, d.WA, cdiv(P, d.WA), (unsigned)p->nx, (unsigned)p->batch))
         return rc;
     hpxfft_dev::GenPlan gB;
@@ -886,6 +1147,7 @@ int run_rows_2level(hpxfft_plan_s *p, const void *real_in, void *real_out, const
         b.out = spec_out;
         b.out_sub = (long long)p->ld_out;
         b.out_bat = (long long)p->dist_out;
+        b.tw2_shift = p->tw2_shift;
         const size_t lds = gen_lds_bytes(p, gB, (size_t)2 * Wh * gB.n);
         const unsigned nt = gen_threads(lds, gB);
         const dim3 grid(cdiv((size_t)Q / 2 + 1, (size_t)Wh), (unsigned)p->nx, (unsigned)p->batch);
@@ -929,10 +1191,10 @@ int run_rows_2level(hpxfft_plan_s *p, const void *real_in, void *real_out, const
         const dim3 grid(cdiv((size_t)L / 2 + 1, 256), (unsigned)p->nx, (unsigned)p->batch);
         if (p->prec == P_F64)
             hipLaunchKernelGGL(hpxfft_dev::gen_split_kernel<double>, grid, dim3(256), 0, p->stream, spec_out,
-                               (long long)p->ld_out, (long long)p->dist_out, (int)L, tab + p->gtab_2);
+                               (long long)p->ld_out, (long long)p->dist_out, (int)L, tab + p->gtab_2, p->tw2_shift);
         else
             hipLaunchKernelGGL(hpxfft_dev::gen_split_kernel<float>, grid, dim3(256), 0, p->stream, spec_out,
-                               (long long)p->ld_out, (long long)p->dist_out, (int)L, tab + p->gtab_2);
+                               (long long)p->ld_out, (long long)p->dist_out, (int)L, tab + p->gtab_2, p->tw2_shift);
         HIPCHK(hipGetLastError());
     }
     return HPXFFT_OK;
@@ -1387,9 +1649,10 @@ int plan_common(hpxfft_plan_s *p)
             return std::string(what) + " = " + std::to_string(n) + " needs at most " + std::to_string(cap) +
                    " points with prime factors <= " + std::to_string(hpxfft_dev::GEN_MAX_RADIX) + " (" +
                    std::to_string(2 * cap) + " with prime factors <= 7), or a Bluestein FFT (the power of two >= 2n-1) of at most " + std::to_string(2 * cap) +
-                   " points, or (up to 2^20 points) a split n = P*Q into two lengths of at most " +
+                   " points, or (up to 2^24 points) a split n = P*Q into two lengths of at most " +
                    std::to_string(cap) + " points, each with such prime factors or a Bluestein FFT of at most " +
-                   std::to_string(cap) + " points, for this precision (got " + dims + ")";
+                   std::to_string(cap) + " points, or (up to 2^22 points) Bluestein's algorithm over two levels, "
+                   "for this precision (got " + dims + ")";
         };
         const size_t wmax = 128 / (2 * storage_bytes(p->prec));   // one 128-byte segment
         if (p->gen_row && !gen_setup(p->ny / 2, cap, p->grow, 2 * cap))
@@ -1408,7 +1671,13 @@ int plan_common(hpxfft_plan_s *p)
                     return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two "
                                                          "only within one workgroup: F64 or F32 for row length "
                                                          "Ny/2 = " + std::to_string(p->ny / 2) + " (got " + dims + ")");
-                return fail(HPXFFT_UNSUPPORTED_SIZE, why("row length Ny/2", p->ny / 2));
+                if (!gen2b_setup(p->ny / 2, cap, p->g2r))
+                    return fail(HPXFFT_UNSUPPORTED_SIZE, why("row length Ny/2", p->ny / 2));
+                if (p->nx > 65535 || p->batch > 65535)
+                    return fail(HPXFFT_UNSUPPORTED_SIZE,
+                                "two-level rows take at most 65535 rows and grids (got " + dims + ")");
+                p->g2r.WA = std::min(gen_width(p->g2r.A, cap, wmax), p->g2r.P);
+                p->g2r.WB = std::min(gen_width(p->g2r.B, cap, wmax), p->g2r.Q);
             }
         }
         if (p->gen_col && !gen_setup(p->nx, cap, p->gcol, 2 * cap))
@@ -1424,7 +1693,10 @@ int plan_common(hpxfft_plan_s *p)
                     return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two "
                                                          "only within one workgroup: F64 or F32 for column length "
                                                          "Nx = " + std::to_string(p->nx) + " (got " + dims + ")");
-                return fail(HPXFFT_UNSUPPORTED_SIZE, why("column length Nx", p->nx));
+                if (!gen2b_setup(p->nx, cap, p->g2c))
+                    return fail(HPXFFT_UNSUPPORTED_SIZE, why("column length Nx", p->nx));
+                p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
+                p->g2c.WB = gen_width(p->g2c.B, cap, wmax);
             }
         }
         if (p->gen_row && !p->g2r.on)
@@ -1466,7 +1738,9 @@ int plan_common(hpxfft_plan_s *p)
     {
         // exact FP64 tables (the long-double evaluation of the power-of-two tables)
         const bool row1 = p->gen_row && !p->g2r.on, col1 = p->gen_col && !p->g2c.on;
-        const size_t L = row1 ? gen_table_size(p->grow) : 0, N2 = p->gen_row ? p->ny : 0;
+        p->tw2_shift = p->gen_row ? split_shift(p->ny) : 0;   // w_Ny, split only for two-level rows
+        const size_t L = row1 ? gen_table_size(p->grow) : 0;
+        const size_t N2 = p->gen_row ? tw_table_size(p->ny, p->tw2_shift) : 0;
         const size_t NX = col1 ? gen_table_size(p->gcol) : 0;
         p->gtab_L = 0;
         p->gtab_2 = L;
@@ -1478,7 +1752,14 @@ int plan_common(hpxfft_plan_s *p)
             d.tabA = total;
             d.tabB = d.tabA + gen_table_size(d.A);
             d.tabT = d.tabB + gen_table_size(d.B);
-            total = d.tabT + (size_t)d.P * (size_t)d.Q;
+            total = d.tabT + tw_table_size(d.points(), d.tw_shift);
+            if (d.blue)
+            {
+                d.tabC = total;
+                total += tw_table_size(2 * d.N, d.chirp_shift);
+                d.tabBh = total;
+                total += d.points();
+            }
         };
         place2(p->g2r);
         place2(p->g2c);
@@ -1492,21 +1773,50 @@ int plan_common(hpxfft_plan_s *p)
                 h[2 * (off + m) + 1] = -(double)sn;
             }
         };
+        auto fill_tw = [&](size_t off, size_t n, int sh) {   // w_n^m, whole or split at 2^sh
+            if (!sh)
+            {
+                fill_w(off, n);
+                return;
+            }
+            const size_t lo = size_t(1) << sh, nhi = (n + lo - 1) >> sh;
+            // lo[i] = w_n^i (i < 2^sh < n), then hi[j] = w_n^{j 2^sh}
+            for (size_t i = 0; i < lo + nhi; ++i)
+            {
+                long double c, sn;
+                twiddle_ld((uint64_t)(i < lo ? i : (i - lo) << sh), n, c, sn);
+                h[2 * (off + i)] = (double)c;
+                h[2 * (off + i) + 1] = -(double)sn;
+            }
+        };
         if (row1) gen_table(p->grow, h, p->gtab_L);
-        if (p->gen_row) fill_w(p->gtab_2, N2);   // w_Ny^m for the r2c split
+        if (p->gen_row) fill_tw(p->gtab_2, p->ny, p->tw2_shift);   // w_Ny^m for the r2c split
         if (col1) gen_table(p->gcol, h, p->gtab_N);
         for (gen2_dim *d : {&p->g2r, &p->g2c})
         {
             if (!d->avail) continue;
             gen_table(d->A, h, d->tabA);
             gen_table(d->B, h, d->tabB);
-            fill_w(d->tabT, (size_t)d->P * (size_t)d->Q);
+            fill_tw(d->tabT, d->points(), d->tw_shift);
+            if (d->blue)
+            {
+                // the chirp's w_{2N} table and Bh = DFT_M(b) / M
+                fill_tw(d->tabC, 2 * d->N, d->chirp_shift);
+                std::vector<long double> cr, ci, br, bi;
+                const uint64_t M = d->points();
+                blue_spectrum(d->N, M, cr, ci, br, bi);
+                for (uint64_t k = 0; k < M; ++k)
+                {
+                    h[2 * (d->tabBh + k)] = (double)(br[k] / (long double)M);
+                    h[2 * (d->tabBh + k) + 1] = (double)(bi[k] / (long double)M);
+                }
+            }
         }
         HIPCHK(hipMalloc(&p->gen_tab, sizeof(double) * h.size()));
         HIPCHK(hipMemcpy(p->gen_tab, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
         const size_t cb = 2 * compute_bytes(p->prec);
-        if (p->g2r.on) HIPCHK(hipMalloc(&p->gtmp, p->batch * p->nx * (p->ny / 2) * cb));
-        if (p->g2c.avail) HIPCHK(hipMalloc(&p->gscr, p->batch * p->nx * p->ncols * cb));
+        if (p->g2r.on) HIPCHK(hipMalloc(&p->gtmp, p->batch * p->nx * p->g2r.points() * cb));
+        if (p->g2c.avail) HIPCHK(hipMalloc(&p->gscr, p->batch * p->g2c.points() * p->ncols * cb));
     }
     if (int rc = choose_columns(p)) return rc;
     if (p->four_step) HIPCHK(hipMalloc(&p->scratch, p->scratch_bytes));

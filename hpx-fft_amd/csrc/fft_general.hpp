@@ -734,6 +734,20 @@ namespace hpxfft_dev {
 // Q*k1 + k2.  gen_strided_kernel runs either phase on W adjacent sequences
 // ("columns") per workgroup, addressed by strides, so the same kernel serves
 // columns of the spectrum and the complex points of a row.
+//
+// Lengths that no split into two one-workgroup lengths covers (a prime
+// factor too large for in-LDS Bluestein) run Bluestein's algorithm with the
+// convolution length M (a power of two >= 2N - 1) as two levels M = P * Q,
+// in three passes over a scratch of M points per sequence:
+//   1. a[m] = x[m] c[m] (0 for m >= N), m = m1 + P m2: Q-point DFTs over m2,
+//      times w_M^{m1 k2}                                   (gen_strided_kernel)
+//   2. for each k2: P-point DFT over m1, times Bh[k2 + Q k1] = DFT_M(b)/M,
+//      inverse P-point DFT over k1, times w_M^{-n1 k2}       (gen_blue_mid_kernel)
+//   3. for each n1: inverse Q-point DFT over k2 -> y[n1 + P n2];
+//      X[n] = c[n] y[n] for n < N                           (gen_strided_kernel)
+// with the chirp c[m] = w_{2N}^{m^2 mod 2N} and b[m] = conj c[|m|] (|m| < N,
+// indices mod M).  Twiddle tables of more than 2^20 entries are split:
+// w_n^m = lo[m mod 2^s] hi[m >> s] (gen_tw), formed in FP64 and rounded once.
 struct GenStride
 {
     const void *in;
@@ -746,7 +760,32 @@ struct GenStride
     int conj_in, conj_out;
     int pre;        // 1: the input is a half-spectrum row G[0..L] turned into the c2r sequence z (2: interior halved)
     int prel;       // pre: L
+    int tw_shift;   // split of the tw_n table (0: one table of tw_n entries)
+    int tw2_shift;  // split of the w_Ny table of the c2r pre-processing and the r2c split
+    // Bluestein passes 1 (blue_in) and 3 (blue_out): element e of sequence f
+    // is index f + blue_p e of the length-blue_n sequence; chirp = w_{2 blue_n}
+    // table split at chirp_shift; blue_conj: conjugate the pass-3 result
+    int blue_in, blue_out, blue_n, blue_p, blue_conj, chirp_shift;
+    const void *chirp;
 };
+
+// w_n^m from a table split at s (lo[i] = w_n^i, i < 2^s, then hi[j] = w_n^{j 2^s}),
+// or from one table when s = 0; FP64 product, rounded once for FP32
+template <typename T>
+__device__ __forceinline__ cpx<T> gen_tw(const cpx<double> *__restrict__ t, int s, long long m)
+{
+    if (s == 0) return tabv<T>(t, (int)m);
+    const cpx<double> w = cmul(t[(1LL << s) + (m >> s)], t[m & ((1LL << s) - 1)]);
+    return cpx<T>{(T)w.x, (T)w.y};
+}
+
+// Bluestein chirp c[m] = w_{2N}^{m^2 mod 2N}
+template <typename T>
+__device__ __forceinline__ cpx<T> gen_chirp(const GenStride &a, long long m)
+{
+    const long long n2 = 2LL * a.blue_n;
+    return gen_tw<T>(reinterpret_cast<const cpx<double> *>(a.chirp), a.chirp_shift, (m * m) % n2);
+}
 
 // element (e, w) of a workgroup's W x n block, for i < n W: sequence-fast
 // order when the sequence stride is 1 (contiguous runs), else column-fast
@@ -782,7 +821,9 @@ __global__ void __launch_bounds__(NTMAX) gen_strided_kernel(GenStride a, GenPlan
         int e, w;
         gen_ew(i, n, W, in_fast, inv_n, inv_w, e, w);
         cpx<T> z = {T(0), T(0)};
-        if (w < nvalid)
+        // Bluestein pass 1: element e of sequence f is index m = f + P e, zero beyond N
+        const long long mb = a.blue_in ? (long long)(a.tw_col ? col0 + w : (int)sub) + (long long)a.blue_p * e : 0;
+        if (w < nvalid && (!a.blue_in || mb < a.blue_n))
         {
             const long long kk = (long long)e * a.in_seq + (long long)(col0 + w) * a.in_col;
             if (!a.pre) z = stor<T>::ld(src + 2 * kk);
@@ -799,13 +840,14 @@ __global__ void __launch_bounds__(NTMAX) gen_strided_kernel(GenStride a, GenPlan
                 {
                     const T hs = a.pre == 2 ? T(0.5) : T(1);
                     const cpx<T> hk = scal(stor<T>::ld(src + 2 * k), hs), hl = scal(stor<T>::ld(src + 2 * (L - k)), hs);
-                    const cpx<T> wv = tabv<T>(tab2, k);
+                    const cpx<T> wv = gen_tw<T>(tab2, a.tw2_shift, k);
                     const cpx<T> wc = {wv.x, -wv.y};
                     const cpx<T> s0 = hk + conjc(hl), d0 = cmul(hk - conjc(hl), wc);
                     z = cpx<T>{s0.x - d0.y, s0.y + d0.x};
                 }
             }
             if (a.conj_in) z = conjc(z);
+            if (a.blue_in) z = cmul(z, gen_chirp<T>(a, mb));
         }
         buf[w * ST + gen_pos(g, tab, e)] = z;
     }
@@ -817,17 +859,73 @@ __global__ void __launch_bounds__(NTMAX) gen_strided_kernel(GenStride a, GenPlan
         gen_ew(i, n, W, out_fast, inv_n, inv_w, k, w);
         if (w >= nvalid) continue;
         cpx<T> z = X[w * ST + k];
+        const int f = a.tw_col ? col0 + w : (int)sub;
         if (a.tw_n > 0)
         {
-            // f < P and k < Q in both uses (m1 k2 < N <= 2^20), so the
+            // f < P and k < Q in every use (m1 k2 < N <= 2^24), so the
             // product is already reduced; the modulo only guards other callers
-            const int f = a.tw_col ? col0 + w : (int)sub;
-            int m = f * k;
+            long long m = (long long)f * k;
             if (m >= a.tw_n) m %= a.tw_n;
-            z = cmul(z, tabv<T>(twt, m));
+            z = cmul(z, gen_tw<T>(twt, a.tw_shift, m));
         }
         if (a.conj_out) z = conjc(z);
+        if (a.blue_out)
+        {
+            // Bluestein pass 3: output k of sequence f is index n = f + P k, kept below N
+            const long long nb = (long long)f + (long long)a.blue_p * k;
+            if (nb >= a.blue_n) continue;
+            z = cmul(z, gen_chirp<T>(a, nb));
+            if (a.blue_conj) z = conjc(z);
+        }
         stor<T>::st(dst + 2 * ((long long)k * a.out_seq + (long long)w * a.out_col), z);
+    }
+}
+
+// Bluestein pass 2 (see GenStride): W sequences of P points (e = m1) per
+// workgroup, sequence f = k2 (the column when tw_col, else blockIdx.y); the
+// P-point DFT, the product with bh[k2 + Q k1], the inverse P-point DFT as
+// conj(DFT(conj .)), then times w_M^{-n1 k2} (twt split at a.tw_shift),
+// written back to the same positions (e = n1).  g: a ping-pong plan.
+template <typename T, int NTMAX = GEN_NT>
+__global__ void __launch_bounds__(NTMAX) gen_blue_mid_kernel(GenStride a, GenPlan g, const cpx<double> *tab,
+                                                             const cpx<double> *bh, const cpx<double> *twt, int W,
+                                                             int Q)
+{
+    cpx<T> *buf = gen_lds<T>();
+    const int t = threadIdx.x, n = g.len, ST = g.n;
+    const int col0 = xcd_order((int)blockIdx.x, (int)gridDim.x) * W;
+    const int nvalid = min(W, a.ncols - col0);
+    const long long sub = blockIdx.y, bat = blockIdx.z;
+    const T *src = reinterpret_cast<const T *>(a.in) + 2 * (bat * a.in_bat + sub * a.in_sub);
+    T *dst = reinterpret_cast<T *>(a.out) + 2 * (bat * a.out_bat + sub * a.out_sub + (long long)col0 * a.out_col);
+    const float inv_n = 1.0f / (float)n, inv_w = 1.0f / (float)W;
+    const bool in_fast = a.in_seq == 1, out_fast = a.out_seq == 1;
+    for (int i = t; i < n * W; i += gen_nth<NTMAX>())
+    {
+        int e, w;
+        gen_ew(i, n, W, in_fast, inv_n, inv_w, e, w);
+        cpx<T> z = {T(0), T(0)};
+        if (w < nvalid) z = stor<T>::ld(src + 2 * ((long long)e * a.in_seq + (long long)(col0 + w) * a.in_col));
+        buf[w * ST + e] = z;
+    }
+    __syncthreads();
+    cpx<T> *F = gen_stages<T, NTMAX>(buf, buf + ST * W, W, g, tab);
+    for (int i = t; i < n * W; i += gen_nth<NTMAX>())
+    {
+        const int w = gdiv(i, n, inv_n), k1 = i - w * n;
+        const long long k2 = a.tw_col ? col0 + w : sub;
+        F[w * ST + k1] = conjc(cmul(F[w * ST + k1], tabv<T>(bh, (int)(k2 + (long long)Q * k1))));
+    }
+    __syncthreads();
+    cpx<T> *G = gen_stages<T, NTMAX>(F, F == buf ? buf + ST * W : buf, W, g, tab);
+    for (int i = t; i < n * W; i += gen_nth<NTMAX>())
+    {
+        int n1, w;
+        gen_ew(i, n, W, out_fast, inv_n, inv_w, n1, w);
+        if (w >= nvalid) continue;
+        const long long k2 = a.tw_col ? col0 + w : sub;
+        const cpx<T> z = cmul(conjc(G[w * ST + n1]), conjc(gen_tw<T>(twt, a.tw_shift, (long long)n1 * k2)));
+        stor<T>::st(dst + 2 * ((long long)n1 * a.out_seq + (long long)w * a.out_col), z);
     }
 }
 
@@ -890,7 +988,7 @@ __global__ void __launch_bounds__(NTMAX) gen_rows_split_kernel(GenStride a, GenP
         const cpx<T> Fe = scal(Zk + Bc, T(0.5));
         const cpx<T> D = scal(Zk - Bc, T(0.5));
         const cpx<T> Fo = {D.y, -D.x};
-        const cpx<T> tt = cmul(tabv<T>(tab2, k), Fo);
+        const cpx<T> tt = cmul(gen_tw<T>(tab2, a.tw2_shift, k), Fo);
         stor<T>::st(dst + 2 * k, Fe + tt);
         stor<T>::st(dst + 2 * (L - k), conjc(Fe - tt));
     }
@@ -902,7 +1000,7 @@ __global__ void __launch_bounds__(NTMAX) gen_rows_split_kernel(GenStride a, GenP
 // blockIdx.y = row, .z = grid; one thread per pair (k, L-k), 2k <= L.
 template <typename T>
 __global__ void __launch_bounds__(256) gen_split_kernel(void *spec, long long row_stride, long long bat_stride, int L,
-                                                         const cpx<double> *tab2)
+                                                         const cpx<double> *tab2, int tw2_shift)
 {
     const int k = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (2 * k > L) return;
@@ -923,7 +1021,7 @@ __global__ void __launch_bounds__(256) gen_split_kernel(void *spec, long long ro
         const cpx<T> Fe = scal(A + Bc, T(0.5));
         const cpx<T> D = scal(A - Bc, T(0.5));
         const cpx<T> Fo = {D.y, -D.x};
-        const cpx<T> tt = cmul(tabv<T>(tab2, k), Fo);
+        const cpx<T> tt = cmul(gen_tw<T>(tab2, tw2_shift, k), Fo);
         stor<T>::st(o + 2 * k, Fe + tt);
         stor<T>::st(o + 2 * (L - k), conjc(Fe - tt));
     }

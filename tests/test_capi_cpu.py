@@ -71,28 +71,41 @@ def plan_status(nx, ny, batch=1, ld_in=None, dist_in=None, ld_out=None, dist_out
     return st
 
 
-@pytest.mark.parametrize("nx,ny", [(16, 1), (16, 15), (0, 16), (16, 0), (1 << 21, 4), (4, 1 << 22),
-                                   (16, 2 * 4099), (4099, 16), (16, 2 * 8193), (2 * 4099, 8),
-                                   ((1 << 20) + 2, 4)])
+@pytest.mark.parametrize("nx,ny", [(16, 1), (16, 15), (0, 16), (16, 0), (1 << 25, 2), (2, 1 << 26),
+                                   (4194319, 2), (2, 2 * 4194319), (2 * 2097169, 2)])
 def test_unsupported_sizes(nx, ny):
-    """Odd Ny, zero sizes, and FP64 lengths no plan covers: a prime factor
-    above 61 whose Bluestein FFT (the power of two >= 2n-1) exceeds 8192
-    points and that no split n = P*Q into lengths of at most 4096 points
-    (Bluestein FFTs of at most 4096) avoids (4099 is prime; 8193 = 3 * 2731,
-    and 2731 needs an 8192-point FFT), or more than 2^20 points (Nx = 2^21,
-    Ny/2 = 2^21)."""
+    """Odd Ny, zero sizes, and lengths no plan covers: more than 2^24 points
+    (Nx = 2^25, Ny/2 = 2^25), and lengths above 2^22 (the longest that
+    Bluestein's algorithm takes over two levels) with no split into two
+    one-workgroup lengths: the prime 4194319, and 2 x the prime 2097169."""
     assert plan_status(nx, ny) == _lib.UNSUPPORTED_SIZE
     assert hpxfft_amd.last_error()
 
 
-def test_fp32_bluestein_reaches_twice_as_far():
-    """FP32 holds twice the points per buffer: Ny/2 = 8191 (prime) fits its
-    16384-point in-place Bluestein FFT, which FP64 (8192 points) rejects."""
-    assert plan_status(16, 2 * 8191) == _lib.UNSUPPORTED_SIZE
-    st = plan_status(16, 2 * 8191, prec=1)
+@pytest.mark.parametrize("nx,ny", [(1 << 21, 4), (4, 1 << 22), (16, 2 * 4099), (4099, 16), (16, 2 * 8193),
+                                   (2 * 4099, 8), ((1 << 20) + 2, 4), (1 << 24, 2), (4194301, 2),
+                                   (2, 2 * 4194301)])
+def test_long_and_large_prime_lengths_pass_validation(nx, ny):
+    """FP64 lengths beyond in-LDS Bluestein with no split into two
+    one-workgroup lengths (4099 and 4194301 are prime; 8193 = 3 * 2731, and
+    2731 needs an 8192-point FFT) run Bluestein's algorithm over two levels;
+    lengths beyond 2^20 with such a split run as two levels (up to 2^24)
+    (without a device the plan then stops at the device check)."""
+    st = plan_status(nx, ny)
     if st == _lib.OK:
         pytest.skip("a HIP device is present")
     assert st == _lib.NO_DEVICE, hpxfft_amd.last_error()
+
+
+def test_fp32_bluestein_reaches_twice_as_far():
+    """FP32 holds twice the points per buffer: Ny/2 = 8191 (prime) fits its
+    16384-point in-place Bluestein FFT; FP64 (8192 points in place) takes it
+    by Bluestein's algorithm over two levels.  Both plans pass validation."""
+    for prec in (0, 1):
+        st = plan_status(16, 2 * 8191, prec=prec)
+        if st == _lib.OK:
+            pytest.skip("a HIP device is present")
+        assert st == _lib.NO_DEVICE, hpxfft_amd.last_error()
 
 
 @pytest.mark.parametrize("prec", [2, 3])
@@ -209,8 +222,9 @@ def test_cpp_harness_argument_errors(tmp_path):
     r = subprocess.run([exe, "--plan=bogus", "--nx", "4", "--ny", "4"], capture_output=True, text=True, timeout=60,
                        cwd=tmp_path)
     assert r.returncode == 2 and "Invalid FFTW plan flag string" in r.stderr
-    r = subprocess.run([exe, "--nx", "8", "--ny", "8198"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
-    assert r.returncode == 2 and "prime factors" in r.stderr       # Ny/2 = 4099, prime
+    r = subprocess.run([exe, "--nx", "1", "--ny", "8388638"], capture_output=True, text=True, timeout=60,
+                       cwd=tmp_path)
+    assert r.returncode == 2 and "prime factors" in r.stderr       # Ny/2 = 4194319, a prime above 2^22
 
 
 @pytest.mark.parametrize("args,msg", [
@@ -237,8 +251,8 @@ def test_cpp_harness_bool_words(tmp_path, word):
     """--header / --result take program_options' bool words, as the
     reference's size sweep passes --header=true.  The unsupported size makes
     the run stop after argument parsing, at plan validation (status 2)."""
-    r = subprocess.run([os.path.join(BIN, "shared_loop_2d_gpu"), "--header=" + word, "--result", word, "--nx=8",
-                        "--ny=8198"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
+    r = subprocess.run([os.path.join(BIN, "shared_loop_2d_gpu"), "--header=" + word, "--result", word, "--nx=1",
+                        "--ny=8388638"], capture_output=True, text=True, timeout=60, cwd=tmp_path)
     assert r.returncode == 2 and "prime factors" in r.stderr and "usage error" not in r.stderr, r.stderr
 
 

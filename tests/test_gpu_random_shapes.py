@@ -6,9 +6,10 @@ up to 3000 (prime lengths among them), and ten with one side of 4097 to
 Bluestein's algorithm and the two-level path all meet.  For each shape the
 plan must either be created, and then match the oracle (FP64 <= 1e-12,
 FP32 <= 1e-5 relative L2, forward; the adjoint on every fourth shape), or
-fail with UNSUPPORTED_SIZE exactly when no plan covers a length: a prime
-factor above 61 that no split N = P*Q avoids and whose Bluestein FFT
-exceeds one workgroup's LDS even in place.
+fail with UNSUPPORTED_SIZE exactly when no plan covers a length (none of
+these sides: lengths up to 2^22 always have a plan, by Bluestein's
+algorithm over two levels when no split N = P*Q and no one-workgroup
+Bluestein FFT covers them).
 """
 import numpy as np
 import pytest
@@ -67,10 +68,11 @@ def _supported(n, cap, pow2_max):
         return True
     if _one_level(n, cap) or (n <= 2 * cap and _largest_prime(n) <= 7):   # in place, one buffer
         return True
-    if n <= 1 << 20 and any(n % p == 0 and n // p >= 2 and _one_level(p, cap) and _one_level(n // p, cap)
+    if n <= 1 << 24 and any(n % p == 0 and n // p >= 2 and _one_level(p, cap) and _one_level(n // p, cap)
                             for p in range(2, min(cap, n // 2) + 1)):
         return True
-    return _bluestein_fft(n) <= 2 * cap      # in place, one buffer of 2 cap points
+    # in place, one buffer of 2 cap points; else Bluestein's algorithm over two levels
+    return _bluestein_fft(n) <= 2 * cap or n <= 1 << 22
 
 
 @pytest.mark.parametrize("nx,ny", SHAPES)

@@ -68,11 +68,11 @@ def test_last_error_is_per_thread(gpu):
 
     bad("main", 0, 16)
     t1 = threading.Thread(target=bad, args=("odd", 16, 15))
-    t2 = threading.Thread(target=bad, args=("prime", 16, 2 * 4099))
+    t2 = threading.Thread(target=bad, args=("prime", 1, 2 * 4194319))   # a prime above 2^22
     t1.start()
     t1.join()
     t2.start()
     t2.join()
     assert "even Ny" in msgs["odd"], msgs
-    assert "4099" in msgs["prime"], msgs
+    assert "4194319" in msgs["prime"], msgs
     assert hpxfft_amd.last_error() == msgs["main"] and "Nx >= 1" in msgs["main"], msgs
