@@ -129,9 +129,9 @@ static int run_all()
         hpxfft::shared::loop f;
         try
         {
-            // Ny = 8198: Ny/2 = 4099 is prime, and its Bluestein FFT (16384 points) exceeds the
-            // 8192 FP64 points one workgroup holds even in place
-            f.initialize(hpxfft::shared::vector_2d(8, 8200, 1.0), "estimate");
+            // Ny = 8388638: Ny/2 = 4194319 is a prime above 2^22, the longest length
+            // Bluestein's algorithm takes over two levels
+            f.initialize(hpxfft::shared::vector_2d(1, 8388640, 1.0), "estimate");
         }
         catch (const std::invalid_argument &)
         {

@@ -253,6 +253,12 @@ struct hpxfft_plan_s
     size_t CW = 0;   // chunk width (columns) of the four-step scratch
     // lengths that are not powers of two: general kernels (F64 / F32 forward)
     bool gen_row = false, gen_col = false;
+    // 16-bit storage with a length beyond one workgroup: the input converted
+    // to an FP32 workspace (in-place padded layout), transformed by an FP32
+    // plan of the same shape, converted back with per-column exponents
+    bool wide16 = false;
+    hpxfft_plan_s *wide = nullptr;
+    float *wide_ws = nullptr;
     hpxfft_dev::GenPlan grow{}, gcol{};
     int gen_w = 1;                 // general column kernel: adjacent columns per workgroup
     int gen_rows = 1;              // general row kernels: rows per workgroup
@@ -726,6 +732,16 @@ std::string describe(const hpxfft_plan_s *p)
     }
     os << "hpxfft-hip plan (gfx950): Nx=" << p->nx << " Ny=" << p->ny << " batch=" << p->batch << " precision "
        << prec_name(p->prec) << " effort " << p->effort << "\n";
+    if (p->wide16)
+    {
+        // the FP32 plan's lines, indented, after the conversion steps
+        os << "  16-bit storage through an FP32 workspace (a length beyond one workgroup): rows converted in "
+              "(times 2^-in_exp), the FP32 plan below in place, column maxima, converted out with per-column "
+              "exponents\n";
+        std::istringstream sub(describe(p->wide));
+        for (std::string ln; std::getline(sub, ln);) os << "    " << ln << "\n";
+        return os.str();
+    }
     auto two_level = [&](const char *what, const gen2_dim &d) {
         if (d.blue)
         {
@@ -1320,6 +1336,8 @@ hipError_t launch_rows_pow2(const hpxfft_plan_s *p, const RowArgs &r, int kind)
     return row_r2c(p->prec, p->lgL, r, gx, (unsigned)p->batch, p->stream);
 }
 
+int exec_wide16(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_exp, int *col_exp);
+
 int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_exp, int *col_exp)
 {
     if (!d_in || !d_out) return fail(HPXFFT_INVALID_ARG, "null device pointer");
@@ -1328,6 +1346,7 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
         return fail(HPXFFT_INVALID_ARG, "in-place exec needs ld_in == 2*ld_out and dist_in == 2*dist_out");
     const bool s16 = p->prec == P_F16 || p->prec == P_BF16;
     if (s16 && !col_exp) col_exp = p->col_exp_own;
+    if (p->wide16) return exec_wide16(p, d_in, d_out, in_exp, col_exp);
     device_guard g(p->device);
     if (p->timed)
     {
@@ -1379,6 +1398,71 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     if (p->timed) HIPCHK(hipEventRecord(p->ev[2], p->stream));
     p->ran = p->timed;
     p->col_split = p->four_step;
+    p->col_first = false;
+    p->h2d = p->d2h = 0.0;
+    return HPXFFT_OK;
+}
+
+// 16-bit storage through the FP32 workspace: convert in (unscaling rows by
+// in_exp), the FP32 plan in place, the column maxima, convert out (scaling
+// columns, exponents to col_exp).  Timing: the FP32 plan records the row /
+// column boundaries into this plan's events, so first_fftw covers the
+// conversion in and the rows, second_fftw the columns and the conversion out.
+int exec_wide16(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_exp, int *col_exp)
+{
+    hpxfft_plan_s *w = p->wide;
+    const long long nc = (long long)p->ncols, ldw = 2 * nc, distw = (long long)p->nx * ldw;
+    const dim3 gin(cdiv(p->ny / 2, 256), (unsigned)p->nx, (unsigned)p->batch);
+    const dim3 gcol(cdiv(p->ncols, 256), (unsigned)p->nx, (unsigned)p->batch);
+    if (p->timed)
+    {
+        if (int rc = alloc_events(p)) return rc;
+        HIPCHK(hipEventRecord(p->ev[0], p->stream));
+    }
+    using namespace hpxfft_dev;
+    if (p->prec == P_F16)
+        hipLaunchKernelGGL(wide_in_kernel<__half>, gin, dim3(256), 0, p->stream, static_cast<const __half *>(d_in),
+                           (long long)p->ld_in, (long long)p->dist_in, in_exp, (long long)p->nx, p->wide_ws, ldw,
+                           distw, (int)(p->ny / 2));
+    else
+        hipLaunchKernelGGL(wide_in_kernel<bf16_t>, gin, dim3(256), 0, p->stream, static_cast<const bf16_t *>(d_in),
+                           (long long)p->ld_in, (long long)p->dist_in, in_exp, (long long)p->nx, p->wide_ws, ldw,
+                           distw, (int)(p->ny / 2));
+    HIPCHK(hipGetLastError());
+    // the FP32 plan records its phase boundaries into this plan's events
+    hipEvent_t own[3] = {w->ev[1], w->ev[2], w->ev[3]};
+    const bool wt = w->timed;
+    if (p->timed)
+    {
+        w->ev[1] = p->ev[1];
+        w->ev[2] = p->ev[2];
+        w->ev[3] = p->ev[3];
+    }
+    w->timed = p->timed;
+    const int rc = exec_forward(w, p->wide_ws, p->wide_ws, nullptr, nullptr);
+    w->ev[1] = own[0];
+    w->ev[2] = own[1];
+    w->ev[3] = own[2];
+    w->timed = wt;
+    if (rc) return rc;
+    const long long ncm = (long long)p->ncols * (long long)p->batch;
+    hipLaunchKernelGGL(zero_u32_kernel<0>, dim3(cdiv((size_t)ncm, 256)), dim3(256), 0, p->stream, p->colmax, ncm);
+    constexpr int ROWS = 64;
+    const dim3 gmax(cdiv(p->ncols, 256), cdiv(p->nx, ROWS), (unsigned)p->batch);
+    hipLaunchKernelGGL(wide_colmax_kernel, gmax, dim3(256), 0, p->stream, p->wide_ws, ldw, distw, (int)p->nx,
+                       (int)p->ncols, ROWS, p->colmax);
+    if (p->prec == P_F16)
+        hipLaunchKernelGGL(wide_out_kernel<__half>, gcol, dim3(256), 0, p->stream, p->wide_ws, ldw, distw,
+                           static_cast<__half *>(d_out), (long long)(2 * p->ld_out), (long long)(2 * p->dist_out),
+                           (int)p->ncols, p->colmax, col_exp, (long long)p->ncols);
+    else
+        hipLaunchKernelGGL(wide_out_kernel<bf16_t>, gcol, dim3(256), 0, p->stream, p->wide_ws, ldw, distw,
+                           static_cast<bf16_t *>(d_out), (long long)(2 * p->ld_out), (long long)(2 * p->dist_out),
+                           (int)p->ncols, p->colmax, col_exp, (long long)p->ncols);
+    HIPCHK(hipGetLastError());
+    if (p->timed) HIPCHK(hipEventRecord(p->ev[2], p->stream));
+    p->ran = p->timed;
+    p->col_split = w->four_step;
     p->col_first = false;
     p->h2d = p->d2h = 0.0;
     return HPXFFT_OK;
@@ -1616,6 +1700,40 @@ int choose_columns(hpxfft_plan_s *p)
     return HPXFFT_OK;
 }
 
+int plan_common(hpxfft_plan_s *p);
+void free_plan(hpxfft_plan_s *p);
+
+// 16-bit storage with a length beyond one workgroup: an FP32 plan of the same
+// shape on an in-place padded FP32 workspace (Nx x 2(Ny/2+1) floats per
+// grid), the column maxima and the caller-less column exponents
+int plan_wide16(hpxfft_plan_s *p)
+{
+    auto *w = new hpxfft_plan_s();
+    w->nx = p->nx;
+    w->ny = p->ny;
+    w->batch = p->batch;
+    w->ld_in = 2 * p->ncols;
+    w->ld_out = p->ncols;
+    w->dist_in = p->nx * w->ld_in;
+    w->dist_out = p->nx * w->ld_out;
+    w->prec = P_F32;
+    w->effort = p->effort;
+    w->stream = p->stream;
+    w->device = p->device;
+    if (int rc = plan_common(w))
+    {
+        free_plan(w);
+        return rc;
+    }
+    p->wide = w;
+    HIPCHK(hipMalloc(&p->wide_ws, sizeof(float) * p->batch * w->dist_in));
+    HIPCHK(hipMalloc(&p->colmax, sizeof(unsigned) * p->ncols * p->batch));
+    HIPCHK(hipMalloc(&p->col_exp_own, sizeof(int) * p->ncols * p->batch));
+    if (int rc = alloc_events(p)) return rc;
+    p->plan_flops = w->plan_flops;
+    return HPXFFT_OK;
+}
+
 int plan_common(hpxfft_plan_s *p)
 {
     const std::string dims = std::to_string(p->nx) + " x " + std::to_string(p->ny);
@@ -1641,81 +1759,110 @@ int plan_common(hpxfft_plan_s *p)
     if (p->gen_row || p->gen_col)
     {
         // lengths that are not powers of two: general kernels (FP64 / FP32;
-        // 16-bit storage within one workgroup, forward only)
+        // 16-bit storage within one workgroup, forward only; 16-bit lengths
+        // beyond one workgroup run through an FP32 workspace, plan_wide16)
         const bool s16 = p->prec == P_F16 || p->prec == P_BF16;
         const size_t cap = p->prec == P_F64 ? hpxfft_dev::gen_max_points<double>()
                                             : hpxfft_dev::gen_max_points<float>();
-        auto why = [&](const char *what, size_t n) {
-            return std::string(what) + " = " + std::to_string(n) + " needs at most " + std::to_string(cap) +
-                   " points with prime factors <= " + std::to_string(hpxfft_dev::GEN_MAX_RADIX) + " (" +
-                   std::to_string(2 * cap) + " with prime factors <= 7), or a Bluestein FFT (the power of two >= 2n-1) of at most " + std::to_string(2 * cap) +
-                   " points, or (up to 2^24 points) a split n = P*Q into two lengths of at most " +
-                   std::to_string(cap) + " points, each with such prime factors or a Bluestein FFT of at most " +
-                   std::to_string(cap) + " points, or (up to 2^22 points) Bluestein's algorithm over two levels, "
-                   "for this precision (got " + dims + ")";
+        hpxfft_dev::GenPlan one{};
+        auto one_level = [&](size_t n) {
+            return gen_setup(n, cap, one, 2 * cap) || gen_setup_in_place_only(n, cap, one);
         };
-        const size_t wmax = 128 / (2 * storage_bytes(p->prec));   // one 128-byte segment
-        if (p->gen_row && !gen_setup(p->ny / 2, cap, p->grow, 2 * cap))
+        p->wide16 = s16 && ((p->gen_row && !one_level(p->ny / 2)) || (p->gen_col && !one_level(p->nx)));
+        if (p->wide16)
         {
-            if (!s16 && gen2_setup(p->ny / 2, cap, p->g2r))
+            // the FP32 plan of plan_wide16 must cover both lengths
+            const size_t c32 = hpxfft_dev::gen_max_points<float>();
+            auto fp32_ok = [&](size_t n) {
+                gen2_dim d;
+                return gen_setup(n, c32, one, 2 * c32) || gen2_setup(n, c32, d) ||
+                       gen_setup_in_place_only(n, c32, one) || gen2b_setup(n, c32, d);
+            };
+            for (int dim = 0; dim < 2; ++dim)
             {
-                if (p->nx > 65535 || p->batch > 65535)
-                    return fail(HPXFFT_UNSUPPORTED_SIZE,
-                                "two-level rows take at most 65535 rows and grids (got " + dims + ")");
-                p->g2r.WA = std::min(gen_width(p->g2r.A, cap, wmax), p->g2r.P);
-                p->g2r.WB = std::min(gen_width(p->g2r.B, cap, wmax), p->g2r.Q);
-            }
-            else if (!gen_setup_in_place_only(p->ny / 2, cap, p->grow))
-            {
-                if (s16)
-                    return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two "
-                                                         "only within one workgroup: F64 or F32 for row length "
-                                                         "Ny/2 = " + std::to_string(p->ny / 2) + " (got " + dims + ")");
-                if (!gen2b_setup(p->ny / 2, cap, p->g2r))
-                    return fail(HPXFFT_UNSUPPORTED_SIZE, why("row length Ny/2", p->ny / 2));
-                if (p->nx > 65535 || p->batch > 65535)
-                    return fail(HPXFFT_UNSUPPORTED_SIZE,
-                                "two-level rows take at most 65535 rows and grids (got " + dims + ")");
-                p->g2r.WA = std::min(gen_width(p->g2r.A, cap, wmax), p->g2r.P);
-                p->g2r.WB = std::min(gen_width(p->g2r.B, cap, wmax), p->g2r.Q);
+                const bool gen = dim ? p->gen_col : p->gen_row;
+                const size_t n = dim ? p->nx : p->ny / 2;
+                if (gen && !fp32_ok(n))
+                    return fail(HPXFFT_UNSUPPORTED_SIZE, std::string(dim ? "column length Nx" : "row length Ny/2") +
+                                                             " = " + std::to_string(n) + " has no FP32 plan (prime "
+                                                             "factors, beyond 2^22 points without a split; got " +
+                                                             dims + ")");
             }
         }
-        if (p->gen_col && !gen_setup(p->nx, cap, p->gcol, 2 * cap))
+        if (!p->wide16)
         {
-            if (!s16 && gen2_setup(p->nx, cap, p->g2c))
+            auto why = [&](const char *what, size_t n) {
+                return std::string(what) + " = " + std::to_string(n) + " needs at most " + std::to_string(cap) +
+                       " points with prime factors <= " + std::to_string(hpxfft_dev::GEN_MAX_RADIX) + " (" +
+                       std::to_string(2 * cap) + " with prime factors <= 7), or a Bluestein FFT (the power of two >= 2n-1) of at most " + std::to_string(2 * cap) +
+                       " points, or (up to 2^24 points) a split n = P*Q into two lengths of at most " +
+                       std::to_string(cap) + " points, each with such prime factors or a Bluestein FFT of at most " +
+                       std::to_string(cap) + " points, or (up to 2^22 points) Bluestein's algorithm over two levels, "
+                       "for this precision (got " + dims + ")";
+            };
+            const size_t wmax = 128 / (2 * storage_bytes(p->prec));   // one 128-byte segment
+            if (p->gen_row && !gen_setup(p->ny / 2, cap, p->grow, 2 * cap))
             {
-                p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
-                p->g2c.WB = gen_width(p->g2c.B, cap, wmax);
+                if (!s16 && gen2_setup(p->ny / 2, cap, p->g2r))
+                {
+                    if (p->nx > 65535 || p->batch > 65535)
+                        return fail(HPXFFT_UNSUPPORTED_SIZE,
+                                    "two-level rows take at most 65535 rows and grids (got " + dims + ")");
+                    p->g2r.WA = std::min(gen_width(p->g2r.A, cap, wmax), p->g2r.P);
+                    p->g2r.WB = std::min(gen_width(p->g2r.B, cap, wmax), p->g2r.Q);
+                }
+                else if (!gen_setup_in_place_only(p->ny / 2, cap, p->grow))
+                {
+                    if (s16)
+                        return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two "
+                                                             "only within one workgroup: F64 or F32 for row length "
+                                                             "Ny/2 = " + std::to_string(p->ny / 2) + " (got " + dims + ")");
+                    if (!gen2b_setup(p->ny / 2, cap, p->g2r))
+                        return fail(HPXFFT_UNSUPPORTED_SIZE, why("row length Ny/2", p->ny / 2));
+                    if (p->nx > 65535 || p->batch > 65535)
+                        return fail(HPXFFT_UNSUPPORTED_SIZE,
+                                    "two-level rows take at most 65535 rows and grids (got " + dims + ")");
+                    p->g2r.WA = std::min(gen_width(p->g2r.A, cap, wmax), p->g2r.P);
+                    p->g2r.WB = std::min(gen_width(p->g2r.B, cap, wmax), p->g2r.Q);
+                }
             }
-            else if (!gen_setup_in_place_only(p->nx, cap, p->gcol))
+            if (p->gen_col && !gen_setup(p->nx, cap, p->gcol, 2 * cap))
             {
-                if (s16)
-                    return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two "
-                                                         "only within one workgroup: F64 or F32 for column length "
-                                                         "Nx = " + std::to_string(p->nx) + " (got " + dims + ")");
-                if (!gen2b_setup(p->nx, cap, p->g2c))
-                    return fail(HPXFFT_UNSUPPORTED_SIZE, why("column length Nx", p->nx));
-                p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
-                p->g2c.WB = gen_width(p->g2c.B, cap, wmax);
+                if (!s16 && gen2_setup(p->nx, cap, p->g2c))
+                {
+                    p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
+                    p->g2c.WB = gen_width(p->g2c.B, cap, wmax);
+                }
+                else if (!gen_setup_in_place_only(p->nx, cap, p->gcol))
+                {
+                    if (s16)
+                        return fail(HPXFFT_UNSUPPORTED_SIZE, "16-bit storage takes lengths that are not powers of two "
+                                                             "only within one workgroup: F64 or F32 for column length "
+                                                             "Nx = " + std::to_string(p->nx) + " (got " + dims + ")");
+                    if (!gen2b_setup(p->nx, cap, p->g2c))
+                        return fail(HPXFFT_UNSUPPORTED_SIZE, why("column length Nx", p->nx));
+                    p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
+                    p->g2c.WB = gen_width(p->g2c.B, cap, wmax);
+                }
             }
-        }
-        if (p->gen_row && !p->g2r.on)
-        {
-            // short rows share a workgroup: about 2048 points per workgroup
-            const size_t ST = (size_t)p->grow.n;
-            p->gen_rows = (int)std::max<size_t>(1, std::min(cap / ST, std::min<size_t>(p->nx, 2048 / ST)));
-        }
-        if (p->gen_col && !p->g2c.on)
-        {
-            // one 128-byte row segment per workgroup where the LDS allows
-            p->gen_w = gen_width(p->gcol, cap, wmax);
-            // narrow strips (16- or 32-byte row segments): at measure effort the
-            // two-level form, with whole segments, is timed against it
-            if (!s16 && p->effort >= HPXFFT_MEASURE && p->gen_w <= 2 && gen2_setup(p->nx, cap, p->g2c))
+            if (p->gen_row && !p->g2r.on)
             {
-                p->g2c.on = false;
-                p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
-                p->g2c.WB = gen_width(p->g2c.B, cap, wmax);
+                // short rows share a workgroup: about 2048 points per workgroup
+                const size_t ST = (size_t)p->grow.n;
+                p->gen_rows = (int)std::max<size_t>(1, std::min(cap / ST, std::min<size_t>(p->nx, 2048 / ST)));
+            }
+            if (p->gen_col && !p->g2c.on)
+            {
+                // one 128-byte row segment per workgroup where the LDS allows
+                p->gen_w = gen_width(p->gcol, cap, wmax);
+                // narrow strips (16- or 32-byte row segments): at measure effort the
+                // two-level form, with whole segments, is timed against it
+                if (!s16 && p->effort >= HPXFFT_MEASURE && p->gen_w <= 2 && gen2_setup(p->nx, cap, p->g2c))
+                {
+                    p->g2c.on = false;
+                    p->g2c.WA = gen_width(p->g2c.A, cap, wmax);
+                    p->g2c.WB = gen_width(p->g2c.B, cap, wmax);
+                }
             }
         }
     }
@@ -1732,6 +1879,7 @@ int plan_common(hpxfft_plan_s *p)
     if (p->dist_in % 2 != 0) return fail(HPXFFT_INVALID_ARG, "dist_in must be even");
     if (int rc = check_device(p->device)) return rc;
     if (int rc = get_tables(p->device, p->tabs)) return rc;
+    if (p->wide16) return plan_wide16(p);
     if (p->gen_row || p->gen_col)
         if (int rc = gen_allow_lds(p->device)) return rc;
     if (p->gen_row || p->gen_col)
@@ -1853,6 +2001,7 @@ void free_plan(hpxfft_plan_s *p)
     if (!p) return;
     device_guard g(p->device);
     for (auto s : p->sub) free_plan(s);
+    free_plan(p->wide);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     else (void)hipDeviceSynchronize();
     for (auto &e : p->ev)
@@ -1867,6 +2016,7 @@ void free_plan(hpxfft_plan_s *p)
     (void)hipFree(p->col_exp_own);
     (void)hipFree(p->adj_work);
     (void)hipFree(p->host_buf);
+    (void)hipFree(p->wide_ws);
     (void)hipFree(p->bgrids);
     for (auto &ph : p->btasks)
         for (auto &t : ph) (void)hipFree(t);

@@ -1027,4 +1027,54 @@ __global__ void __launch_bounds__(256) gen_split_kernel(void *spec, long long ro
     }
 }
 
+// 16-bit storage with a length beyond one workgroup runs an FP32 plan on a
+// workspace: these kernels convert into it and back.  blockIdx.y = row,
+// .z = grid; one thread per complex pair.
+// in: Ny/2 pairs of a 16-bit row (times 2^-in_exp[row] when given) -> FP32
+template <typename S>
+__global__ void __launch_bounds__(256) wide_in_kernel(const S *in, long long ld_in, long long dist_in,
+                                                       const int *in_exp, long long exp_dist, float *ws,
+                                                       long long ld_ws, long long dist_ws, int pairs)
+{
+    const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (j >= pairs) return;
+    const long long b = blockIdx.z, row = blockIdx.y;
+    cpx<float> z = stor<S>::ld(in + b * dist_in + row * ld_in + 2 * j);
+    if (in_exp) z = scal(z, ldexpf(1.0f, -in_exp[b * exp_dist + row]));
+    stor<float>::st(ws + b * dist_ws + row * ld_ws + 2 * j, z);
+}
+
+// the largest component of each spectrum column over `rows` rows per block
+// (colmax cleared beforehand; float bits, so the unsigned maximum orders them)
+__global__ void __launch_bounds__(256) wide_colmax_kernel(const float *ws, long long ld_ws, long long dist_ws,
+                                                           int nrows, int ncols, int rows, unsigned *colmax)
+{
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= ncols) return;
+    const long long b = blockIdx.z;
+    const int r0 = (int)blockIdx.y * rows, r1 = min(nrows, r0 + rows);
+    float m = 0.0f;
+    for (int r = r0; r < r1; ++r)
+    {
+        const cpx<float> z = stor<float>::ld(ws + b * dist_ws + (long long)r * ld_ws + 2 * c);
+        m = fmaxf(m, fmaxf(fabsf(z.x), fabsf(z.y)));
+    }
+    atomicMax(&colmax[b * ncols + c], __float_as_uint(m));
+}
+
+// out: X * 2^e per column, e = pick_exp(column maximum), e -> col_exp
+template <typename S>
+__global__ void __launch_bounds__(256) wide_out_kernel(const float *ws, long long ld_ws, long long dist_ws, S *out,
+                                                        long long ld_out, long long dist_out, int ncols,
+                                                        const unsigned *colmax, int *col_exp, long long cexp_dist)
+{
+    const int c = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= ncols) return;
+    const long long b = blockIdx.z, row = blockIdx.y;
+    const int e = pick_exp(__uint_as_float(colmax[b * ncols + c]));
+    const cpx<float> z = stor<float>::ld(ws + b * dist_ws + row * ld_ws + 2 * c);
+    stor<S>::st(out + b * dist_out + row * ld_out + 2 * c, scal(z, ldexpf(1.0f, e)));
+    if (row == 0 && col_exp) col_exp[b * cexp_dist + c] = e;
+}
+
 }   // namespace hpxfft_dev

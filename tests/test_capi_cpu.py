@@ -109,19 +109,20 @@ def test_fp32_bluestein_reaches_twice_as_far():
 
 
 @pytest.mark.parametrize("prec", [2, 3])
-def test_general_lengths_16bit_within_one_workgroup(prec):
+def test_general_lengths_16bit(prec):
     """16-bit storage takes lengths that are not powers of two within one
-    workgroup (here 8 x 14, Bluestein lengths, 8191, whose Bluestein FFT
-    runs in place only, and 10000 = 2^4 5^4, in place in one buffer);
-    lengths that need two levels (20000) ask for an F64 or F32 plan."""
-    for nx, ny in [(8, 14), (67, 16), (16, 134), (8191, 8), (8, 2 * 8191), (10000, 8)]:
+    workgroup (8 x 14, Bluestein lengths, 8191, whose Bluestein FFT runs in
+    place only, and 10000 = 2^4 5^4, in place in one buffer), and beyond one
+    workgroup (20000 as two levels, the prime 8209 by Bluestein's algorithm
+    over two levels, 2^17) through an FP32 workspace; lengths no FP32 plan
+    covers are refused."""
+    for nx, ny in [(8, 14), (67, 16), (16, 134), (8191, 8), (8, 2 * 8191), (10000, 8), (20000, 8),
+                   (8, 2 * 20000), (8209, 4), (4, 1 << 18)]:
         st = plan_status(nx, ny, prec=prec)
         if st == _lib.OK:
             pytest.skip("a HIP device is present")
-        assert st == _lib.NO_DEVICE, hpxfft_amd.last_error()
-    for nx, ny in [(20000, 8), (8, 2 * 20000)]:
-        assert plan_status(nx, ny, prec=prec) == _lib.UNSUPPORTED_SIZE
-        assert "F64 or F32" in hpxfft_amd.last_error() and "16-bit" in hpxfft_amd.last_error()
+        assert st == _lib.NO_DEVICE, (nx, ny, hpxfft_amd.last_error())
+    assert plan_status(4194319, 2, prec=prec) == _lib.UNSUPPORTED_SIZE
 
 
 @pytest.mark.parametrize("nx,ny", [(8, 14), (3, 16), (1000, 64), (64, 1000), (4095, 2),

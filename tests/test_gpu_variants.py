@@ -2,7 +2,8 @@
 
 * FP16 / BF16 storage with FP32 arithmetic and power-of-two scales: the
   spectrum reconstructed as X16 * 2^-col_exp is compared with the FP64
-  oracle of the original (unrounded) input.  Bounds: FP16 rel-L2 <= 5e-3,
+  oracle of the original (unrounded) input, at lengths within one workgroup
+  and beyond it (an FP32 workspace).  Bounds: FP16 rel-L2 <= 5e-3,
   BF16 <= 4e-2 (11- and 8-bit significands; these variants report their
   error rather than meet the FP64 bar).  Inputs of magnitude 1e6 exercise the
   input row scales (they would overflow FP16 unscaled).
@@ -47,7 +48,12 @@ def to16(x_pad, ny, dt, scale_rows):
                                        # one LDS buffer, in place: 12000 = 2^5 3 5^3, and 8191 (prime)
                                        # through a 16384-point Bluestein FFT; 4093 through 8192 points
                                        (12000, 16, 1e6), (8, 2 * 12000, 1.0), (8191, 8, 1.0),
-                                       (16, 2 * 4093, 1e6)])
+                                       (16, 2 * 4093, 1e6),
+                                       # beyond one workgroup: through an FP32 workspace (two levels,
+                                       # Bluestein over two levels for the prime 8209, powers of two
+                                       # beyond the power-of-two kernels)
+                                       (20000, 16, 1e6), (16, 2 * 20000, 1.0), (8209, 8, 1.0),
+                                       (4, 2 * 8209, 1e6), (1 << 17, 4, 1.0), (4, 1 << 17, 1.0)])
 def test_16bit_storage(gpu, dt_name, bound, nx, ny, amp):
     import torch
     import hpxfft_amd
@@ -65,6 +71,17 @@ def test_16bit_storage(gpu, dt_name, bound, nx, ny, amp):
     err = oracle.rel_l2(Xc[:, : ny // 2 + 1], ref)
     print(f"  rel-L2 vs FP64 oracle = {err:.3e}", flush=True)
     assert err <= bound, err
+
+
+@pytest.mark.parametrize("dt_name", ["f16", "bf16"])
+def test_16bit_beyond_one_workgroup_plan(gpu, dt_name):
+    """16-bit plans with a length beyond one workgroup say how they run: the
+    FP32 plan on a workspace, its own two-level / Bluestein lines indented."""
+    import hpxfft_amd
+    p = hpxfft_amd.Plan(20000, 16, precision=dt_name)
+    d = p.describe()
+    p.close()
+    assert "FP32 workspace" in d and "precision f32" in d and "two levels" in d, d
 
 
 def test_16bit_row_exponents_are_validated(gpu):
