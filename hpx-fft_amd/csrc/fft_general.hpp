@@ -206,8 +206,9 @@ __device__ void gen_stage_reg(const cpx<T> *A, cpx<T> *B, int nseq, int n, int N
 // one in-place decimation-in-time stage of radix R (2, 3, 4, 5, 7, 8) over nseq
 // sequences: butterfly (c, j) reads and writes the R positions
 // c R Ls + j + q Ls, so the butterflies of a stage touch disjoint points
-template <typename T, int R, int NTMAX>
-__device__ void gen_stage_ip(cpx<T> *A, int nseq, int n, int Ls, const cpx<double> *__restrict__ tab)
+template <typename T, int R, int NTMAX, bool POST = false>
+__device__ void gen_stage_ip(cpx<T> *A, int nseq, int n, int Ls, const cpx<double> *__restrict__ tab,
+                             const cpx<double> *__restrict__ chirp = nullptr, int len = 0)
 {
     const int t = threadIdx.x, m = n / R, nb = nseq * m, tstep = n / (R * Ls);
     const float inv_m = 1.0f / (float)m, inv_ls = 1.0f / (float)Ls;
@@ -253,9 +254,55 @@ __device__ void gen_stage_ip(cpx<T> *A, int nseq, int n, int Ls, const cpx<doubl
             }
             dft_small<T, R>(x[u], w);
             cpx<T> *p = A + base[u];
+            if constexpr (POST)
+            {
+                // the last stage (R Ls = n, c = 0): output e = jj + q Ls, conjugated
+                // and times the chirp below len (the rest is never read)
 #pragma unroll
-            for (int q = 0; q < R; ++q) p[q * Ls] = x[u][q];
+                for (int q = 0; q < R; ++q)
+                {
+                    const int e = jj[u] + q * Ls;
+                    p[q * Ls] = e < len ? cmul(conjc(x[u][q]), tabv<T>(chirp, e)) : x[u][q];
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < R; ++q) p[q * Ls] = x[u][q];
+            }
         }
+    }
+    __syncthreads();
+}
+
+// Bluestein's middle pass, in place: the last decimation-in-frequency stage
+// (radix R on blocks of R points, no twiddles), the product with the kernel
+// spectrum in the stages' position order, conjugated, and the first
+// decimation-in-time stage (radix R on the same blocks, no twiddles), in
+// registers: two LDS passes and two barriers fewer than running them apart
+template <typename T, int R, int NTMAX>
+__device__ void gen_stage_blue_mid(cpx<T> *A, int nseq, int n, const cpx<double> *__restrict__ tab,
+                                   const cpx<double> *__restrict__ bhp)
+{
+    const int t = threadIdx.x, per = n / R, nb = nseq * per;
+    const float inv_per = 1.0f / (float)per;
+    cpx<T> w[R];
+#pragma unroll
+    for (int e = 1; e < R; ++e) w[e] = tabv<T>(tab, e * per);
+    w[0] = cpx<T>{T(1), T(0)};
+    for (int b = t; b < nb; b += gen_nth<NTMAX>())
+    {
+        const int seq = gdiv(b, per, inv_per), c = b - seq * per;
+        cpx<T> *p = A + seq * n + c * R;
+        cpx<T> x[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) x[q] = p[q];
+        dft_small<T, R>(x, w);
+#pragma unroll
+        for (int q = 0; q < R; ++q) x[q] = conjc(cmul(x[q], tabv<T>(bhp, c * R + q)));
+        dft_small<T, R>(x, w);
+#pragma unroll
+        for (int q = 0; q < R; ++q) p[q] = x[q];
     }
     __syncthreads();
 }
@@ -324,8 +371,9 @@ __device__ cpx<T> *gen_stages(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, 
 // one in-place decimation-in-frequency stage of radix R on blocks of Lb
 // points: butterfly (c, j < Lb/R) reads the R positions c Lb + j + q Lb/R,
 // transforms them and multiplies output p by w_{Lb}^{j p}
-template <typename T, int R, int NTMAX>
-__device__ void gen_stage_dif(cpx<T> *A, int nseq, int n, int Lb, const cpx<double> *__restrict__ tab)
+template <typename T, int R, int NTMAX, bool PRE = false>
+__device__ void gen_stage_dif(cpx<T> *A, int nseq, int n, int Lb, const cpx<double> *__restrict__ tab,
+                              const cpx<double> *__restrict__ chirp = nullptr, int len = 0)
 {
     const int t = threadIdx.x, per = n / R, nb = nseq * per, m = Lb / R, tstep = n / Lb;
     const float inv_per = 1.0f / (float)per, inv_m = 1.0f / (float)m;
@@ -348,8 +396,22 @@ __device__ void gen_stage_dif(cpx<T> *A, int nseq, int n, int Lb, const cpx<doub
                 const int seq = gdiv(b, per, inv_per), r = b - seq * per;
                 const int c = gdiv(r, m, inv_m), j = r - c * m;
                 base[u] = seq * n + c * Lb + j;
+                if constexpr (PRE)
+                {
+                    // the first stage (Lb = n, c = 0): element e = j + q m, times the
+                    // chirp, zero beyond len (those points were never loaded)
 #pragma unroll
-                for (int q = 0; q < R; ++q) x[u][q] = A[base[u] + q * m];
+                    for (int q = 0; q < R; ++q)
+                    {
+                        const int e = j + q * m;
+                        x[u][q] = e < len ? cmul(A[base[u] + q * m], tabv<T>(chirp, e)) : cpx<T>{T(0), T(0)};
+                    }
+                }
+                else
+                {
+#pragma unroll
+                    for (int q = 0; q < R; ++q) x[u][q] = A[base[u] + q * m];
+                }
                 w1[u] = tab[j * tstep];
                 jj[u] = j;
             }
@@ -422,6 +484,72 @@ __device__ void gen_stages_ip(cpx<T> *A, int nseq, const GenPlan &g, const cpx<d
     }
 }
 
+// In-place Bluestein convolution of a power-of-two plan (radices 2, 4, 8),
+// S >= 2 stages: the chirp product fused into the first DIF stage, the
+// kernel-spectrum product between the last DIF and the first DIT stage
+// (gen_stage_blue_mid), the conjugation and output chirp into the last DIT
+// stage; natural order in and out
+template <typename T, int NTMAX>
+__device__ void gen_blue_ip(cpx<T> *A, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
+{
+    const cpx<double> *chirp = tab + g.off_c, *bhp = tab + g.off_bp;
+    const int S = g.nstages, n = g.n, len = g.len;
+    int Lb = n;
+    for (int s = S - 1; s >= 1; --s)
+    {
+        const int r = g.radix[s];
+        if (s == S - 1)
+        {
+            switch (r)
+            {
+            case 2: gen_stage_dif<T, 2, NTMAX, true>(A, nseq, n, Lb, tab, chirp, len); break;
+            case 4: gen_stage_dif<T, 4, NTMAX, true>(A, nseq, n, Lb, tab, chirp, len); break;
+            default: gen_stage_dif<T, 8, NTMAX, true>(A, nseq, n, Lb, tab, chirp, len); break;
+            }
+        }
+        else
+        {
+            switch (r)
+            {
+            case 2: gen_stage_dif<T, 2, NTMAX>(A, nseq, n, Lb, tab); break;
+            case 4: gen_stage_dif<T, 4, NTMAX>(A, nseq, n, Lb, tab); break;
+            default: gen_stage_dif<T, 8, NTMAX>(A, nseq, n, Lb, tab); break;
+            }
+        }
+        Lb /= r;
+    }
+    switch (g.radix[0])
+    {
+    case 2: gen_stage_blue_mid<T, 2, NTMAX>(A, nseq, n, tab, bhp); break;
+    case 4: gen_stage_blue_mid<T, 4, NTMAX>(A, nseq, n, tab, bhp); break;
+    default: gen_stage_blue_mid<T, 8, NTMAX>(A, nseq, n, tab, bhp); break;
+    }
+    int Ls = g.radix[0];
+    for (int s = 1; s < S; ++s)
+    {
+        const int r = g.radix[s];
+        if (s == S - 1)
+        {
+            switch (r)
+            {
+            case 2: gen_stage_ip<T, 2, NTMAX, true>(A, nseq, n, Ls, tab, chirp, len); break;
+            case 4: gen_stage_ip<T, 4, NTMAX, true>(A, nseq, n, Ls, tab, chirp, len); break;
+            default: gen_stage_ip<T, 8, NTMAX, true>(A, nseq, n, Ls, tab, chirp, len); break;
+            }
+        }
+        else
+        {
+            switch (r)
+            {
+            case 2: gen_stage_ip<T, 2, NTMAX>(A, nseq, n, Ls, tab); break;
+            case 4: gen_stage_ip<T, 4, NTMAX>(A, nseq, n, Ls, tab); break;
+            default: gen_stage_ip<T, 8, NTMAX>(A, nseq, n, Ls, tab); break;
+            }
+        }
+        Ls *= r;
+    }
+}
+
 // The DFT of length g.len of nseq sequences held at A[s*g.n + e], e < g.len.
 // Direct plans (g.len == g.n) run the Stockham stages.  Otherwise Bluestein's
 // algorithm: with the chirp c[j] = w_{2 len}^{j^2} (tab + off_c),
@@ -440,6 +568,11 @@ __device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, con
         return A;
     }
     if (g.len == g.n) return gen_stages<T, NTMAX>(A, B, nseq, g, tab);
+    if (g.ip && g.nstages >= 2)
+    {
+        gen_blue_ip<T, NTMAX>(A, nseq, g, tab);
+        return A;
+    }
     const int t = threadIdx.x, n = g.n, len = g.len, total = nseq * n;
     const cpx<double> *chirp = tab + g.off_c, *bh = tab + g.off_b;
     const float inv_n = 1.0f / (float)n;
