@@ -465,10 +465,12 @@ __device__ void gen_stages_dif(cpx<T> *A, int nseq, const GenPlan &g, const cpx<
 // all stages of an in-place plan (radices <= 8) on the digit-reversed
 // sequences in A; the result is in A in natural order
 template <typename T, int NTMAX>
-__device__ void gen_stages_ip(cpx<T> *A, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
+__device__ void gen_stages_ip(cpx<T> *A, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab,
+                              int first = 0)
 {
     int Ls = 1;
-    for (int s = 0; s < g.nstages; ++s)
+    for (int s = 0; s < first; ++s) Ls *= g.radix[s];
+    for (int s = first; s < g.nstages; ++s)
     {
         const int r = g.radix[s];
         switch (r)
@@ -560,11 +562,12 @@ __device__ void gen_blue_ip(cpx<T> *A, int nseq, const GenPlan &g, const cpx<dou
 // [s*g.n + k], k < g.len.  In-place plans (g.ip) take A loaded in the
 // gen_pos() order and return A.
 template <typename T, int NTMAX>
-__device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab)
+__device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab,
+                           int first = 0)
 {
     if (g.ip && g.len == g.n)
     {
-        gen_stages_ip<T, NTMAX>(A, nseq, g, tab);
+        gen_stages_ip<T, NTMAX>(A, nseq, g, tab, first);
         return A;
     }
     if (g.len == g.n) return gen_stages<T, NTMAX>(A, B, nseq, g, tab);
@@ -622,6 +625,69 @@ __device__ cpx<T> *gen_fft(cpx<T> *A, cpx<T> *B, int nseq, const GenPlan &g, con
 
 __device__ __forceinline__ float absmax(cpx<float> z) { return fmaxf(fabsf(z.x), fabsf(z.y)); }
 
+// Direct in-place plans: the loads fused with the first decimation-in-time
+// stage.  The elements whose first digit d_1 is 0 are e0 < n/r_1, and
+// e0 + q n/r_1 (q < r_1) sit at positions pos(e0) + q: one block of the
+// first stage (radix r_1, no twiddles).  A thread loads the r_1 elements,
+// transforms them and writes the block, one LDS pass and barrier fewer than
+// scattering the loads and running the stage; gen_fft then starts at stage 1.
+// ld(s, e) is element e of sequence s (zero outside the data); s_fast: the
+// sequence index runs fastest across threads (adjacent columns), else the
+// element index (a row's contiguous points).
+template <typename T, int R, int NTMAX, typename LD>
+__device__ void gen_load_first_r(cpx<T> *buf, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab,
+                                 bool s_fast, LD ld)
+{
+    const int n = g.n, per = n / R, total = nseq * per;
+    cpx<T> w[R];
+#pragma unroll
+    for (int e = 1; e < R; ++e) w[e] = tabv<T>(tab, e * per);
+    w[0] = cpx<T>{T(1), T(0)};
+    const float inv_per = 1.0f / (float)per, inv_ns = 1.0f / (float)nseq;
+    for (int i = threadIdx.x; i < total; i += gen_nth<NTMAX>())
+    {
+        int s, e0;
+        if (s_fast)
+        {
+            e0 = gdiv(i, nseq, inv_ns);
+            s = i - e0 * nseq;
+        }
+        else
+        {
+            s = gdiv(i, per, inv_per);
+            e0 = i - s * per;
+        }
+        cpx<T> x[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) x[q] = ld(s, e0 + q * per);
+        dft_small<T, R>(x, w);
+        cpx<T> *o = buf + s * n + gen_pos(g, tab, e0);
+#pragma unroll
+        for (int q = 0; q < R; ++q) o[q] = x[q];
+    }
+}
+
+// whether a plan takes the fused loads (direct, in place)
+__device__ __forceinline__ bool gen_fuses_first(const GenPlan &g)
+{
+    return g.ip && g.len == g.n && g.off_r >= 0 && g.nstages >= 1;
+}
+
+template <typename T, int NTMAX, typename LD>
+__device__ void gen_load_first(cpx<T> *buf, int nseq, const GenPlan &g, const cpx<double> *__restrict__ tab,
+                               bool s_fast, LD ld)
+{
+    switch (g.radix[0])
+    {
+    case 2: gen_load_first_r<T, 2, NTMAX>(buf, nseq, g, tab, s_fast, ld); break;
+    case 3: gen_load_first_r<T, 3, NTMAX>(buf, nseq, g, tab, s_fast, ld); break;
+    case 4: gen_load_first_r<T, 4, NTMAX>(buf, nseq, g, tab, s_fast, ld); break;
+    case 5: gen_load_first_r<T, 5, NTMAX>(buf, nseq, g, tab, s_fast, ld); break;
+    case 8: gen_load_first_r<T, 8, NTMAX>(buf, nseq, g, tab, s_fast, ld); break;
+    default: gen_load_first_r<T, 7, NTMAX>(buf, nseq, g, tab, s_fast, ld); break;   // in-place radices are 2..8
+    }
+}
+
 template <typename T> __device__ __forceinline__ cpx<T> *gen_lds()
 {
     extern __shared__ __align__(16) unsigned char gen_smem[];
@@ -656,9 +722,7 @@ __global__ void __launch_bounds__(NTMAX) gen_row_r2c_kernel(RowArgs a, GenPlan g
         for (int i = t; i < nseq; i += gen_nth<NTMAX>()) smax[i] = 0u;   // ordered before use by the barriers below
     }
     const float inv_l = 1.0f / (float)L;
-    for (int i = t; i < nseq * L; i += gen_nth<NTMAX>())
-    {
-        const int s = gdiv(i, L, inv_l), e = i - s * L;
+    auto load = [&](int s, int e) {
         cpx<T> z = {T(0), T(0)};
         if (s < nrow)
         {
@@ -668,10 +732,20 @@ __global__ void __launch_bounds__(NTMAX) gen_row_r2c_kernel(RowArgs a, GenPlan g
                 if (a.in_exp) z = scal(z, (T)ldexpf(1.0f, -a.in_exp[b * a.exp_dist + row0 + s]));
             }
         }
-        buf[s * ST + gen_pos(g, tabL, e)] = z;
+        return z;
+    };
+    const bool fused = gen_fuses_first(g);
+    if (fused) gen_load_first<T, NTMAX>(buf, nseq, g, tabL, false, load);
+    else
+    {
+        for (int i = t; i < nseq * L; i += gen_nth<NTMAX>())
+        {
+            const int s = gdiv(i, L, inv_l), e = i - s * L;
+            buf[s * ST + gen_pos(g, tabL, e)] = load(s, e);
+        }
     }
     __syncthreads();
-    const cpx<T> *Zs = gen_fft<T, NTMAX>(buf, buf + nseq * ST, nseq, g, tabL);
+    const cpx<T> *Zs = gen_fft<T, NTMAX>(buf, buf + nseq * ST, nseq, g, tabL, fused ? 1 : 0);
     const int half = L / 2 + 1;   // k = 0 .. floor(L/2)
     const float inv_h = 1.0f / (float)half;
     // X[k] and X[L-k] of row s (X[0] and X[L] for k = 0; X[L/2] alone for 2k = L)
@@ -763,10 +837,8 @@ __global__ void __launch_bounds__(NTMAX) gen_col_kernel(ColArgs a, GenPlan g, co
         if (t < W) smax[t] = 0u;   // ordered before use by the barriers below
     }
     const float inv_w = 1.0f / (float)W;
-    // row segments: element (e, w) -> buf[w * ST + e]
-    for (int i = t; i < n * W; i += gen_nth<NTMAX>())
-    {
-        const int e = gdiv(i, W, inv_w), w = i - e * W;
+    // row segments: element (e, w) -> buf[w * ST + pos(e)]
+    auto load = [&](int w, int e) {
         cpx<T> z = {T(0), T(0)};
         if (w < nvalid)
         {
@@ -776,10 +848,20 @@ __global__ void __launch_bounds__(NTMAX) gen_col_kernel(ColArgs a, GenPlan g, co
                 if (a.row_exp) z = scal(z, (T)ldexpf(1.0f, -a.row_exp[b * a.rexp_dist + e]));
             }
         }
-        buf[w * ST + gen_pos(g, tab, e)] = CONJ ? conjc(z) : z;
+        return CONJ ? conjc(z) : z;
+    };
+    const bool fused = gen_fuses_first(g);
+    if (fused) gen_load_first<T, NTMAX>(buf, W, g, tab, true, load);
+    else
+    {
+        for (int i = t; i < n * W; i += gen_nth<NTMAX>())
+        {
+            const int e = gdiv(i, W, inv_w), w = i - e * W;
+            buf[w * ST + gen_pos(g, tab, e)] = load(w, e);
+        }
     }
     __syncthreads();
-    const cpx<T> *X = gen_fft<T, NTMAX>(buf, buf + ST * W, W, g, tab);
+    const cpx<T> *X = gen_fft<T, NTMAX>(buf, buf + ST * W, W, g, tab, fused ? 1 : 0);
     if constexpr (S16)
     {
         for (int i = t; i < n * W; i += gen_nth<NTMAX>())
