@@ -488,4 +488,169 @@ row_r2c_big_kernel(RowArgs a)
                             reinterpret_cast<const cpx<double> *>(a.tw64), orow, t, &smax, oexp);
 }
 
+// Backward rows of 2^11, 2^13 and 2^14 complex points on the radix-16 stages
+// of the forward kernel: the row half of the adjoint (HALVE) and the inverse
+// c2r.  From the half spectrum G[0..L] of a row:
+//   H = G on 0<k<L (halved when HALVE), H[0] = Re G[0], H[L] = Re G[L],
+//   Z[k] = (H[k] + conj H[L-k]) + i (H[k] - conj H[L-k]) conj(w_{2L}^k),
+// and the output row is conj(DFT(conj Z)) as interleaved reals.  Each thread
+// builds the pairs (k, L-k) of its tasks into LDS (loads of G, G[L-k] and the
+// twiddle all issued before the first wait), the stages run as in the forward
+// kernel, and the last stage is a plain radix-RL Stockham stage whose outputs
+// j + q L/RL are stored as they are produced.  Workgroup g takes row g.
+// waves per SIMD of the backward long rows: as the forward rows, and FP64
+// rows of 8192 points held to 128 VGPRs so that two workgroups share a CU
+template <typename T, int LG> constexpr int c2r_min_waves()
+{
+    return sizeof(T) == 8 && LG == 13 ? 4 : big_min_waves<T, LG>();
+}
+
+template <typename T, typename S, int LG, bool HALVE>
+__global__ void __launch_bounds__((1 << LG) / (16 * big_nb<T, LG>()), (c2r_min_waves<T, LG>()))
+row_c2r_big_kernel(RowArgs a)
+{
+    constexpr int L = 1 << LG;
+    constexpr int NB = big_nb<T, LG>();
+    using B = bigrow<T, L, NB>;
+    constexpr int NT = B::NT, RL = B::RL, NSL = B::NSL, M = B::M;
+    constexpr int HK = L / 2 + 1;                      // pair tasks k = 0 .. L/2
+    constexpr int NTASK = (HK + NT - 1) / NT;
+    constexpr int NF = NSL / NT;                       // last-stage butterflies per thread
+    static_assert(NSL % NT == 0, "last-stage butterflies divide among the threads");
+    __shared__ T lds[B::SPLIT ? B::LP : 2 * B::LP];
+    const int t = threadIdx.x;
+    const int g = xcd_order((int)blockIdx.x, (int)gridDim.x);   // consecutive rows share an XCD
+    const int bat = g / a.nrows, row = g - bat * a.nrows;
+    const S *irow = reinterpret_cast<const S *>(a.in) + (long long)bat * a.dist_in + (long long)row * a.ld_in;
+    S *orow = reinterpret_cast<S *>(a.out) + (long long)bat * a.dist_out + (long long)row * a.ld_out;
+    const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(a.tw);
+    const cpx<double> *tw64 = reinterpret_cast<const cpx<double> *>(a.tw64);
+
+    // pair tasks in chunks of CH: each chunk's loads of G[k], G[L-k] and the
+    // twiddle are issued together (tasks past L/2 read the pair L/2 and write
+    // nothing); split rows build one component per pass, reading G again
+    constexpr int CH = 4;
+    auto pair = [&](int k, const cpx<T> &g0, const cpx<T> &g1, const cpx<T> &w, cpx<T> &z0, cpx<T> &z1) {
+        if (k == 0)
+        {
+            // H0 = Re G0, HL = Re GL: Z0 = (H0 + HL) + i (H0 - HL)
+            z0 = conjc(cpx<T>{g0.x + g1.x, g0.x - g1.x});
+            z1 = z0;
+        }
+        else
+        {
+            const T hs = HALVE ? T(0.5) : T(1);
+            const cpx<T> h0 = scal(g0, hs), h1 = scal(g1, hs);
+            const cpx<T> s0 = h0 + conjc(h1), d0 = cmul(h0 - conjc(h1), conjc(w));
+            const cpx<T> s1 = h1 + conjc(h0), d1 = cmul(h1 - conjc(h0), cpx<T>{-w.x, -w.y});   // conj w^{L-k} = -w^k
+            z0 = conjc(cpx<T>{s0.x - d0.y, s0.y + d0.x});
+            z1 = conjc(cpx<T>{s1.x - d1.y, s1.y + d1.x});
+        }
+    };
+    cpx<T> v[16 * NB];
+#pragma unroll
+    for (int h = 0; h < (B::SPLIT ? 2 : 1); ++h)
+    {
+#pragma unroll 1
+        for (int c0 = 0; c0 < NTASK; c0 += CH)
+        {
+            cpx<T> G0[CH], G1[CH], W[CH];
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+            {
+                const int k = min(t + (c0 + i) * NT, L / 2);
+                G0[i] = stor<S>::ld(irow + 2 * k);
+                G1[i] = stor<S>::ld(irow + 2 * (L - k));
+                W[i] = tw[k];
+            }
+#pragma unroll
+            for (int i = 0; i < CH; ++i)
+            {
+                const int k = t + (c0 + i) * NT;
+                if (c0 + i >= NTASK || k > L / 2) continue;
+                cpx<T> z0, z1;
+                pair(k, G0[i], G1[i], W[i], z0, z1);
+                if constexpr (!B::SPLIT)
+                {
+                    cpx<T> *l = reinterpret_cast<cpx<T> *>(lds);
+                    l[B::pad(k)] = z0;
+                    if (k != 0) l[B::pad(L - k)] = z1;
+                }
+                else
+                {
+                    lds[B::pad(k)] = h ? z0.y : z0.x;
+                    if (k != 0) lds[B::pad(L - k)] = h ? z1.y : z1.x;
+                }
+            }
+        }
+        __syncthreads();
+        // each thread's stage-0 inputs v[16 b + r] = z[t + b NT + r M]
+#pragma unroll
+        for (int u = 0; u < 16 * NB; ++u)
+        {
+            const int e = B::pad(t + (u / 16) * NT + (u % 16) * M);
+            if constexpr (!B::SPLIT) v[u] = reinterpret_cast<const cpx<T> *>(lds)[e];
+            else if (h) v[u].y = lds[e];
+            else v[u].x = lds[e];
+        }
+        __syncthreads();
+    }
+    radix16_stages<T, L, NB, 0>(v, lds, tw64, t);
+    constexpr int NS = B::ns(B::NR16 - 1);
+    constexpr int WS = NS == 1 ? 1 : NS + NS / 16;
+    constexpr int RS = NSL + NSL / 16;
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+    {
+        const int k = (t + b * NT) & (NS - 1);
+        if constexpr (NS > 1) twiddle16<T, L, NS>(v + 16 * b, tw64, k);
+        dft16(v + 16 * b);
+    }
+    // the last radix-16 stage's outputs to LDS; the radix-RL stage reads d[j + r NSL]
+    cpx<T> f[NF][RL];
+#pragma unroll
+    for (int h = 0; h < (B::SPLIT ? 2 : 1); ++h)
+    {
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+        {
+            const int j = t + b * NT, k = j & (NS - 1);
+            const int wb = B::pad((j / NS) * 16 * NS + k);
+#pragma unroll
+            for (int q = 0; q < 16; ++q)
+            {
+                if constexpr (!B::SPLIT) reinterpret_cast<cpx<T> *>(lds)[wb + q * WS] = v[16 * b + q];
+                else lds[wb + q * WS] = h ? v[16 * b + q].y : v[16 * b + q].x;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NF; ++i)
+        {
+            const int pa = B::pad(t + i * NT);
+#pragma unroll
+            for (int r = 0; r < RL; ++r)
+            {
+                if constexpr (!B::SPLIT) f[i][r] = reinterpret_cast<const cpx<T> *>(lds)[pa + r * RS];
+                else if (h) f[i][r].y = lds[pa + r * RS];
+                else f[i][r].x = lds[pa + r * RS];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < NF; ++i)
+    {
+        const int j = t + i * NT;
+        // twiddles w_L^{r j} = w_{2L}^{2 r j}
+#pragma unroll
+        for (int r = 1; r < RL; ++r) f[i][r] = cmul(f[i][r], tw[2 * r * j]);
+        dftR<T, RL>(f[i]);
+        // u = DFT(conj Z); the row is conj(u): out[2e] = Re u, out[2e+1] = -Im u
+#pragma unroll
+        for (int q = 0; q < RL; ++q)
+            stor<S>::st(orow + 2 * (j + q * NSL), cpx<T>{f[i][q].x, -f[i][q].y});
+    }
+}
+
 }   // namespace hpxfft_dev

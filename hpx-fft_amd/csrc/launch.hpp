@@ -172,7 +172,16 @@ hipError_t launch_row_r2c_pair(int lg, const RowArgs &a, unsigned total, hipStre
 template <typename T, typename S, int LG>
 hipError_t launch_row_c2r(bool halve, const RowArgs &a, unsigned gx, unsigned gy, hipStream_t s)
 {
-    if (halve)
+    if constexpr (use_big_row<T, S, LG>())
+    {
+        // long rows: the radix-16 stages of the forward kernel, one row per workgroup
+        (void)gx;
+        const unsigned total = (unsigned)a.nrows * gy;
+        const dim3 nt((1 << LG) / (16 * big_nb<T, LG>()));
+        if (halve) hipLaunchKernelGGL((row_c2r_big_kernel<T, S, LG, true>), dim3(total), nt, 0, s, a);
+        else hipLaunchKernelGGL((row_c2r_big_kernel<T, S, LG, false>), dim3(total), nt, 0, s, a);
+    }
+    else if (halve)
         hipLaunchKernelGGL((row_c2r_kernel<T, S, S, LG, true>), dim3(gx, gy), dim3(cfg<T, S>::row_NT(LG)), 0, s, a);
     else
         hipLaunchKernelGGL((row_c2r_kernel<T, S, S, LG, false>), dim3(gx, gy), dim3(cfg<T, S>::row_NT(LG)), 0, s, a);
