@@ -488,6 +488,8 @@ LEGS = {
     "mixed_2e15": bench_mixed,
     "general_length": bench_general,
     "general_length_bluestein": lambda: bench_general(2039, 2 * 2039),
+    "general_length_long": lambda: bench_general(6000, 12000),
+    "general_length_two_level": lambda: bench_general(9000, 18000),
 }
 
 

@@ -363,8 +363,10 @@ int gen_allow_lds(int dev)
         GEN_BOTH(gen_col_kernel, float, __half, false), GEN_BOTH(gen_col_kernel, float, bf16_t, false),
         GEN_BOTH(gen_row_c2r_kernel, double, false), GEN_BOTH(gen_row_c2r_kernel, double, true),
         GEN_BOTH(gen_row_c2r_kernel, float, false), GEN_BOTH(gen_row_c2r_kernel, float, true),
-        GEN_BOTH(gen_strided_kernel, double), GEN_BOTH(gen_strided_kernel, float),
-        GEN_BOTH(gen_rows_split_kernel, double), GEN_BOTH(gen_rows_split_kernel, float),
+        GEN_BOTH(gen_strided_kernel, double, false), GEN_BOTH(gen_strided_kernel, float, false),
+        GEN_BOTH(gen_strided_kernel, double, true), GEN_BOTH(gen_strided_kernel, float, true),
+        GEN_BOTH(gen_rows_split_kernel, double, false), GEN_BOTH(gen_rows_split_kernel, float, false),
+        GEN_BOTH(gen_rows_split_kernel, double, true), GEN_BOTH(gen_rows_split_kernel, float, true),
         GEN_BOTH(gen_blue_mid_kernel, double), GEN_BOTH(gen_blue_mid_kernel, float)};
 #undef GEN_BOTH
     // the buffers plus up to 1024 words of per-row / per-column maxima
@@ -870,11 +872,19 @@ int launch_strided(hpxfft_plan_s *p, const hpxfft_dev::GenStride &a, const hpxff
     const size_t lds = gen_lds_bytes(p, g, (size_t)W * g.n);
     const dim3 grid(gx, gy, gz);
     const unsigned nt = gen_threads(lds, g);
-    if (p->prec == P_F64)
-        GEN_LAUNCH(hpxfft_dev::gen_strided_kernel, (double), nt, grid, lds, p->stream, a, g, tab + tab_off,
+    // the general form only where a split table or a Bluestein pass needs it
+    const bool general = a.tw_shift || a.tw2_shift || a.blue_in || a.blue_out;
+    if (p->prec == P_F64 && general)
+        GEN_LAUNCH(hpxfft_dev::gen_strided_kernel, (double, true), nt, grid, lds, p->stream, a, g, tab + tab_off,
+                   tab + twt_off, tab + p->gtab_2, W);
+    else if (p->prec == P_F64)
+        GEN_LAUNCH(hpxfft_dev::gen_strided_kernel, (double, false), nt, grid, lds, p->stream, a, g, tab + tab_off,
+                   tab + twt_off, tab + p->gtab_2, W);
+    else if (general)
+        GEN_LAUNCH(hpxfft_dev::gen_strided_kernel, (float, true), nt, grid, lds, p->stream, a, g, tab + tab_off,
                    tab + twt_off, tab + p->gtab_2, W);
     else
-        GEN_LAUNCH(hpxfft_dev::gen_strided_kernel, (float), nt, grid, lds, p->stream, a, g, tab + tab_off,
+        GEN_LAUNCH(hpxfft_dev::gen_strided_kernel, (float, false), nt, grid, lds, p->stream, a, g, tab + tab_off,
                    tab + twt_off, tab + p->gtab_2, W);
     HIPCHK(hipGetLastError());
     return HPXFFT_OK;
@@ -1168,12 +1178,18 @@ int run_rows_2level(hpxfft_plan_s *p, const void *real_in, void *real_out, const
         const unsigned nt = gen_threads(lds, gB);
         const dim3 grid(cdiv((size_t)Q / 2 + 1, (size_t)Wh), (unsigned)p->nx, (unsigned)p->batch);
         const auto *tab = reinterpret_cast<const hpxfft_dev::cpx<double> *>(p->gen_tab);
-        if (p->prec == P_F64)
-            GEN_LAUNCH(hpxfft_dev::gen_rows_split_kernel, (double), nt, grid, lds, p->stream, b, gB, tab + d.tabB,
-                       tab + p->gtab_2, Wh, (int)Q);
+        if (p->prec == P_F64 && b.tw2_shift)
+            GEN_LAUNCH(hpxfft_dev::gen_rows_split_kernel, (double, true), nt, grid, lds, p->stream, b, gB,
+                       tab + d.tabB, tab + p->gtab_2, Wh, (int)Q);
+        else if (p->prec == P_F64)
+            GEN_LAUNCH(hpxfft_dev::gen_rows_split_kernel, (double, false), nt, grid, lds, p->stream, b, gB,
+                       tab + d.tabB, tab + p->gtab_2, Wh, (int)Q);
+        else if (b.tw2_shift)
+            GEN_LAUNCH(hpxfft_dev::gen_rows_split_kernel, (float, true), nt, grid, lds, p->stream, b, gB,
+                       tab + d.tabB, tab + p->gtab_2, Wh, (int)Q);
         else
-            GEN_LAUNCH(hpxfft_dev::gen_rows_split_kernel, (float), nt, grid, lds, p->stream, b, gB, tab + d.tabB,
-                       tab + p->gtab_2, Wh, (int)Q);
+            GEN_LAUNCH(hpxfft_dev::gen_rows_split_kernel, (float, false), nt, grid, lds, p->stream, b, gB,
+                       tab + d.tabB, tab + p->gtab_2, Wh, (int)Q);
         HIPCHK(hipGetLastError());
         return HPXFFT_OK;
     }

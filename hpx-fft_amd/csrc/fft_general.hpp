@@ -1018,7 +1018,9 @@ __device__ __forceinline__ void gen_ew(int i, int n, int W, bool seq_fast, float
     }
 }
 
-template <typename T, int NTMAX = GEN_NT>
+// GX: the general form (split twiddle tables, Bluestein passes 1 and 3);
+// without it those paths are compiled out (the plain two-level phases)
+template <typename T, bool GX = false, int NTMAX = GEN_NT>
 __global__ void __launch_bounds__(NTMAX) gen_strided_kernel(GenStride a, GenPlan g, const cpx<double> *tab,
                                                             const cpx<double> *twt, const cpx<double> *tab2, int W)
 {
@@ -1037,8 +1039,9 @@ __global__ void __launch_bounds__(NTMAX) gen_strided_kernel(GenStride a, GenPlan
         gen_ew(i, n, W, in_fast, inv_n, inv_w, e, w);
         cpx<T> z = {T(0), T(0)};
         // Bluestein pass 1: element e of sequence f is index m = f + P e, zero beyond N
-        const long long mb = a.blue_in ? (long long)(a.tw_col ? col0 + w : (int)sub) + (long long)a.blue_p * e : 0;
-        if (w < nvalid && (!a.blue_in || mb < a.blue_n))
+        const bool blue_in = GX && a.blue_in;
+        const long long mb = blue_in ? (long long)(a.tw_col ? col0 + w : (int)sub) + (long long)a.blue_p * e : 0;
+        if (w < nvalid && (!blue_in || mb < a.blue_n))
         {
             const long long kk = (long long)e * a.in_seq + (long long)(col0 + w) * a.in_col;
             if (!a.pre) z = stor<T>::ld(src + 2 * kk);
@@ -1055,14 +1058,14 @@ __global__ void __launch_bounds__(NTMAX) gen_strided_kernel(GenStride a, GenPlan
                 {
                     const T hs = a.pre == 2 ? T(0.5) : T(1);
                     const cpx<T> hk = scal(stor<T>::ld(src + 2 * k), hs), hl = scal(stor<T>::ld(src + 2 * (L - k)), hs);
-                    const cpx<T> wv = gen_tw<T>(tab2, a.tw2_shift, k);
+                    const cpx<T> wv = gen_tw<T>(tab2, GX ? a.tw2_shift : 0, k);
                     const cpx<T> wc = {wv.x, -wv.y};
                     const cpx<T> s0 = hk + conjc(hl), d0 = cmul(hk - conjc(hl), wc);
                     z = cpx<T>{s0.x - d0.y, s0.y + d0.x};
                 }
             }
             if (a.conj_in) z = conjc(z);
-            if (a.blue_in) z = cmul(z, gen_chirp<T>(a, mb));
+            if (blue_in) z = cmul(z, gen_chirp<T>(a, mb));
         }
         buf[w * ST + gen_pos(g, tab, e)] = z;
     }
@@ -1078,13 +1081,14 @@ __global__ void __launch_bounds__(NTMAX) gen_strided_kernel(GenStride a, GenPlan
         if (a.tw_n > 0)
         {
             // f < P and k < Q in every use (m1 k2 < N <= 2^24), so the
-            // product is already reduced; the modulo only guards other callers
-            long long m = (long long)f * k;
+            // product is already reduced and fits an int; the modulo only
+            // guards other callers
+            int m = f * k;
             if (m >= a.tw_n) m %= a.tw_n;
-            z = cmul(z, gen_tw<T>(twt, a.tw_shift, m));
+            z = cmul(z, gen_tw<T>(twt, GX ? a.tw_shift : 0, m));
         }
         if (a.conj_out) z = conjc(z);
-        if (a.blue_out)
+        if (GX && a.blue_out)
         {
             // Bluestein pass 3: output k of sequence f is index n = f + P k, kept below N
             const long long nb = (long long)f + (long long)a.blue_p * k;
@@ -1155,7 +1159,7 @@ __global__ void __launch_bounds__(NTMAX) gen_blue_mid_kernel(GenStride a, GenPla
 // self-paired column (k2 = 0, or Q/2) writes the pairs with 2k <= L.
 // tab: phase B's plan table; tab2: w_{Ny}^m.  Dynamic LDS: gen_nbuf(g) * 2 Wh
 // * g.n complex.
-template <typename T, int NTMAX = GEN_NT>
+template <typename T, bool GX = false, int NTMAX = GEN_NT>
 __global__ void __launch_bounds__(NTMAX) gen_rows_split_kernel(GenStride a, GenPlan g, const cpx<double> *tab,
                                                                const cpx<double> *tab2, int Wh, int Q)
 {
@@ -1203,7 +1207,7 @@ __global__ void __launch_bounds__(NTMAX) gen_rows_split_kernel(GenStride a, GenP
         const cpx<T> Fe = scal(Zk + Bc, T(0.5));
         const cpx<T> D = scal(Zk - Bc, T(0.5));
         const cpx<T> Fo = {D.y, -D.x};
-        const cpx<T> tt = cmul(gen_tw<T>(tab2, a.tw2_shift, k), Fo);
+        const cpx<T> tt = cmul(gen_tw<T>(tab2, GX ? a.tw2_shift : 0, k), Fo);
         stor<T>::st(dst + 2 * k, Fe + tt);
         stor<T>::st(dst + 2 * (L - k), conjc(Fe - tt));
     }
