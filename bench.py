@@ -407,7 +407,7 @@ def bench_c1(nx: int = 1 << 8, ny: int = 1 << 14, seed: int = 1, reps: int = 50,
 
 def bench_general(nx: int = 3000, ny: int = 3000, reps: int = 10) -> dict:
     """Lengths that are not powers of two, FP64, on the general kernels
-    (2-pass byte model): by default 3000 x 3000 (radices 4, 2, 3, 5); with
+    (2-pass byte model): by default 3000 x 3000 (3000 = 2^3 3 5^3); with
     prime lengths such as 2039 x 4078 the Bluestein path; 6000 x 12000 runs
     its long lengths (radices 2..7) in place in one LDS buffer, and lengths
     beyond that such as 9000 x 18000 take the two-level path (two launches
