@@ -9,6 +9,7 @@ namespace hpxfft_host {
 
 constexpr int BATCH_NCFG = 4;   // candidate configurations per phase
 constexpr int BATCH_MIN_SIDE = 64, BATCH_MAX_SIDE = 2048;
+constexpr int BATCH_QUEUES = 8;   // ticket counters per phase (one task range per XCD); `ticket` points at them
 
 // phase 0 = rows, 1 = columns
 int batched_pts(int phase, int cfg);

@@ -302,7 +302,7 @@ struct hpxfft_plan_s
     int bblocks[2][hpxfft_host::BATCH_NCFG] = {};
     double btune_ms[2][hpxfft_host::BATCH_NCFG] = {};
     double split_ms[COL_TILE_MAX_LG + 1] = {};   // four-step: column-pass ms per tried log2 P (effort >= measure)
-    unsigned *bticket = nullptr;                            // two work-queue counters
+    unsigned *bticket = nullptr;                            // work-queue counters: BATCH_QUEUES per phase
     void *bstage = nullptr;                                 // pinned staging of the grid descriptors, two slots
     hipEvent_t bstage_ev[2] = {nullptr, nullptr};           // each slot's last upload (the slot is free once it ran)
     int bstage_next = 0;                                    // slot of the next upload
@@ -2448,12 +2448,14 @@ int batched_run(hpxfft_plan_s *p, int cfg_row, int cfg_col)
     const int cfg[2] = {cfg_row, cfg_col};
     for (int ph = 0; ph < 2; ++ph)
     {
-        hipLaunchKernelGGL(hpxfft_dev::zero_u32_kernel<0>, dim3(1), dim3(64), 0, p->stream, p->bticket + ph, 1LL);
+        unsigned *tk = p->bticket + ph * BATCH_QUEUES;
+        hipLaunchKernelGGL(hpxfft_dev::zero_u32_kernel<0>, dim3(1), dim3(64), 0, p->stream, tk,
+                           (long long)BATCH_QUEUES);
         HIPCHK(hipGetLastError());
         const int c = cfg[ph];
         const int nb = std::min(p->bblocks[ph][c], std::max(1, p->bntasks[ph][c]));
-        HIPCHK(batched_launch(ph, c, p->bgrids, p->btasks[ph][c], p->bntasks[ph][c], p->bticket + ph, p->tabs.tw32,
-                              nb, p->stream));
+        HIPCHK(batched_launch(ph, c, p->bgrids, p->btasks[ph][c], p->bntasks[ph][c], tk, p->tabs.tw32, nb,
+                              p->stream));
         if (ph == 0) HIPCHK(hipEventRecord(p->ev[1], p->stream));
     }
     return HPXFFT_OK;
@@ -2521,11 +2523,12 @@ int batched_autotune(hpxfft_plan_s *p)
                 const int reps = 2 + tune_reps(p->effort);   // two warm-up runs, then timed
                 for (int rep = 0; rep < reps && !rc; ++rep)
                 {
-                    hipLaunchKernelGGL(hpxfft_dev::zero_u32_kernel<0>, dim3(1), dim3(64), 0, p->stream,
-                                       p->bticket + ph, 1LL);
+                    unsigned *tk = p->bticket + ph * BATCH_QUEUES;
+                    hipLaunchKernelGGL(hpxfft_dev::zero_u32_kernel<0>, dim3(1), dim3(64), 0, p->stream, tk,
+                                       (long long)BATCH_QUEUES);
                     (void)hipEventRecord(a, p->stream);
-                    if (batched_launch(ph, c, p->bgrids, p->btasks[ph][c], p->bntasks[ph][c], p->bticket + ph,
-                                       p->tabs.tw32, nb, p->stream) != hipSuccess)
+                    if (batched_launch(ph, c, p->bgrids, p->btasks[ph][c], p->bntasks[ph][c], tk, p->tabs.tw32, nb,
+                                       p->stream) != hipSuccess)
                         rc = fail(HPXFFT_HIP_ERROR, "batched autotune launch failed");
                     (void)hipEventRecord(b, p->stream);
                     (void)hipEventSynchronize(b);
@@ -2571,7 +2574,7 @@ int plan_batched(hpxfft_plan_s *p)
         HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         HIPCHK(hipEventRecord(e, p->stream));   // both slots start free
     }
-    HIPCHK(hipMalloc(&p->bticket, 2 * sizeof(unsigned)));
+    HIPCHK(hipMalloc(&p->bticket, 2 * BATCH_QUEUES * sizeof(unsigned)));
     const size_t tb = batched_task_desc_bytes();
     for (int ph = 0; ph < 2; ++ph)
     {

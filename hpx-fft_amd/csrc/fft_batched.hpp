@@ -7,7 +7,13 @@
 // same number of points, PTS: PTS/L rows of a grid with row length L =
 // Ny/2, or a strip of PTS/Nx adjacent columns of a grid with Nx rows, so
 // uneven grids still give equal-sized work items.  Workgroups take tasks
-// from a device ticket counter until the task table is exhausted.
+// from device ticket counters until the task table is exhausted.  Column
+// strips of one grid share 128-byte lines with their neighbours, so the
+// column table (neighbouring strips next to each other) is cut into 8
+// contiguous ranges, one per XCD (workgroup i runs on XCD i mod 8), and
+// neighbours run behind one L2; a workgroup whose range is exhausted takes
+// tasks from the other ranges.  Row tasks share no lines and take tickets
+// from one counter.
 //
 // Candidate configurations (the autotuner times them on the plan's grids):
 //   rows:    CFG 0: PTS 4096, 16 points/thread, 256 threads
@@ -61,9 +67,35 @@ template <int CFG> constexpr int bcol_lds()
     return bcol_cfg<CFG>::PTS + bcol_cfg<CFG>::PTS / 16;
 }
 
-__device__ __forceinline__ int take_ticket(unsigned *ticket, unsigned *shared_slot)
+constexpr int BATCH_QUEUES = 8;   // ticket counters per phase: one task range per XCD
+
+// the next task for this workgroup, or ntasks when every range is exhausted.
+// NQ ranges (1 or BATCH_QUEUES): its own XCD's range first (counter x, tasks
+// [ntasks x / NQ, ntasks (x+1) / NQ)), then the others in turn.  A counter
+// past its range stays past it, so each workgroup ends after at most one
+// miss per range.
+template <int NQ>
+__device__ __forceinline__ int take_ticket(unsigned *ticket, int ntasks, unsigned *shared_slot)
 {
-    if (threadIdx.x == 0) *shared_slot = atomicAdd(ticket, 1u);
+    if (threadIdx.x == 0)
+    {
+        const int x0 = (int)(blockIdx.x % NQ);
+        int t = ntasks;
+        for (int d = 0; d < NQ; ++d)
+        {
+            const int x = (x0 + d) % NQ;
+            const int lo = (int)((long long)ntasks * x / NQ);
+            const int hi = (int)((long long)ntasks * (x + 1) / NQ);
+            if (lo >= hi) continue;
+            const unsigned k = atomicAdd(&ticket[x], 1u);
+            if (k < (unsigned)(hi - lo))
+            {
+                t = lo + (int)k;
+                break;
+            }
+        }
+        *shared_slot = (unsigned)t;
+    }
     __syncthreads();
     const int t = (int)*shared_slot;
     __syncthreads();
@@ -100,7 +132,7 @@ batched_rows_kernel(const BGrid *__restrict__ grids, const BTask *__restrict__ t
     __shared__ unsigned slot;
     for (;;)
     {
-        const int t = take_ticket(ticket, &slot);
+        const int t = take_ticket<1>(ticket, ntasks, &slot);
         if (t >= ntasks) break;
         const BTask task = tasks[t];
         const BGrid g = grids[task.grid];
@@ -145,7 +177,7 @@ batched_cols_kernel(const BGrid *__restrict__ grids, const BTask *__restrict__ t
     __shared__ unsigned slot;
     for (;;)
     {
-        const int t = take_ticket(ticket, &slot);
+        const int t = take_ticket<BATCH_QUEUES>(ticket, ntasks, &slot);
         if (t >= ntasks) break;
         const BTask task = tasks[t];
         const BGrid g = grids[task.grid];

@@ -6,6 +6,8 @@ namespace hpxfft_host {
 
 using namespace hpxfft_dev;
 
+static_assert(hpxfft_host::BATCH_QUEUES == hpxfft_dev::BATCH_QUEUES, "ticket counters per phase");
+
 int batched_pts(int phase, int cfg)
 {
     if (phase == 0)
