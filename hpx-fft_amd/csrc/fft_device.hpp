@@ -238,12 +238,14 @@ template <> struct stor<__half>
     }
 };
 __device__ __forceinline__ float bf16_to_f(uint32_t h) { return __uint_as_float(h << 16); }
-__device__ __forceinline__ uint32_t f_to_bf16(float f)
+// two floats -> two BF16 (round to nearest even) packed low/high in one word:
+// one v_cvt_pk_bf16_f32 on gfx950
+typedef float f32x2_v __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2_v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t f2_to_bf16x2(float lo, float hi)
 {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);   // inf / nan
-    u += 0x7fffu + ((u >> 16) & 1u);                                                        // round to nearest even
-    return u >> 16;
+    const f32x2_v f = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(f, bf16x2_v));
 }
 template <> struct stor<bf16_t>
 {
@@ -254,7 +256,7 @@ template <> struct stor<bf16_t>
     }
     __device__ static void st(bf16_t *p, cpx<float> v)
     {
-        *reinterpret_cast<uint32_t *>(p) = f_to_bf16(v.x) | (f_to_bf16(v.y) << 16);
+        *reinterpret_cast<uint32_t *>(p) = f2_to_bf16x2(v.x, v.y);
     }
 };
 

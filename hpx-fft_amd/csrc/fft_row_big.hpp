@@ -11,10 +11,10 @@
 // real and imaginary halves separately so the LDS footprint is 8 bytes per
 // point (two workgroups per CU at L = 8192).
 //
-// 16-bit storage (FP32 arithmetic): inputs are converted on load and scaled
-// by 2^-in_exp[row] when given; the row's outputs stay in registers until the
-// workgroup has their largest component, then are stored as X * 2^e with
-// e = pick_exp(max) written to out_exp[row].
+// 16-bit storage (FP32 arithmetic, stage twiddles formed in FP32): inputs
+// are converted on load and scaled by 2^-in_exp[row] when given; the row's
+// outputs stay in registers until the workgroup has their largest component,
+// then are stored as X * 2^e with e = pick_exp(max) written to out_exp[row].
 #pragma once
 
 #include "fft_kernels.hpp"
@@ -49,17 +49,20 @@ template <typename T, int L, int NB = 1> struct bigrow
 template <typename T, int LG> constexpr int big_nb() { return sizeof(T) == 4 && LG >= 13 ? 2 : 1; }
 template <typename T, int LG> constexpr int big_min_waves() { return big_nb<T, LG>() == 2 ? 4 : 1; }
 
-// stage twiddles w_{16 NS}^{r k}, r = 1..15, from the FP64 w_{2L} table:
-// table values w, w^2, w^4, w^8 and products of at most 3 factors, formed in
-// FP64 and (for FP32 rows) rounded once -- the only FP64 step of the FP32 path.
-template <typename T, int L, int NS>
-__device__ __forceinline__ void twiddle16(cpx<T> *v, const cpx<double> *__restrict__ tw64, int k)
+// stage twiddles w_{16 NS}^{r k}, r = 1..15, from the w_{2L} table: table
+// values w, w^2, w^4, w^8 and products of at most 3 factors.  W = double: the
+// FP64 table, products formed in FP64 and (for FP32 rows) rounded once -- the
+// only FP64 step of the FP32 path.  W = T = float (rows stored in 16 bits):
+// the FP32 table, products in FP32 (a few FP32 ulps, far below the 16-bit
+// rounding of the outputs) at half the issue cost of the FP64 products.
+template <typename T, int L, int NS, typename W = double>
+__device__ __forceinline__ void twiddle16(cpx<T> *v, const cpx<W> *__restrict__ tab, int k)
 {
     const int base = k * (L / (8 * NS));   // index of w_{16 NS}^k in the w_{2L} table
-    const cpx<double> w1 = tw64[base], w2 = tw64[2 * base], w4 = tw64[4 * base], w8 = tw64[8 * base];
-    const cpx<double> w3 = cmul(w2, w1), w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
-    // each factor is applied as soon as it is formed (fewer live FP64 registers)
-    auto app = [&](int r, const cpx<double> &w) { v[r] = cmul(v[r], cpx<T>{(T)w.x, (T)w.y}); };
+    const cpx<W> w1 = tab[base], w2 = tab[2 * base], w4 = tab[4 * base], w8 = tab[8 * base];
+    const cpx<W> w3 = cmul(w2, w1), w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
+    // each factor is applied as soon as it is formed (fewer live registers)
+    auto app = [&](int r, const cpx<W> &w) { v[r] = cmul(v[r], cpx<T>{(T)w.x, (T)w.y}); };
     app(1, w1);
     app(2, w2);
     app(3, w3);
@@ -77,8 +80,10 @@ __device__ __forceinline__ void twiddle16(cpx<T> *v, const cpx<double> *__restri
     app(15, cmul(w8, w7));
 }
 
-template <typename T, int L, int NB, int S>
-__device__ __forceinline__ void radix16_stages(cpx<T> *v, T *lds, const cpx<double> *__restrict__ tw64, int t)
+// TW32: stage twiddles from the FP32 table tw (16-bit storage), else from tw64
+template <typename T, int L, int NB, int S, bool TW32 = false>
+__device__ __forceinline__ void radix16_stages(cpx<T> *v, T *lds, const cpx<double> *__restrict__ tw64, int t,
+                                               const cpx<T> *__restrict__ tw = nullptr)
 {
     using B = bigrow<T, L, NB>;
     // stages 0 .. NR16-2 (each followed by an exchange); stage NR16-1 is
@@ -94,7 +99,11 @@ __device__ __forceinline__ void radix16_stages(cpx<T> *v, T *lds, const cpx<doub
         for (int b = 0; b < NB; ++b)
         {
             const int j = t + b * B::NT, k = j & (NS - 1);
-            if constexpr (NS > 1) twiddle16<T, L, NS>(v + 16 * b, tw64, k);
+            if constexpr (NS > 1)
+            {
+                if constexpr (TW32) twiddle16<T, L, NS, T>(v + 16 * b, tw, k);
+                else twiddle16<T, L, NS>(v + 16 * b, tw64, k);
+            }
             dft16(v + 16 * b);
             wb[b] = B::pad((j / NS) * 16 * NS + k);
             rb[b] = B::pad(j);
@@ -136,7 +145,7 @@ __device__ __forceinline__ void radix16_stages(cpx<T> *v, T *lds, const cpx<doub
                 for (int r = 0; r < 16; ++r) v[16 * b + r].y = lds[rb[b] + r * RS];
             __syncthreads();
         }
-        radix16_stages<T, L, NB, S + 1>(v, lds, tw64, t);
+        radix16_stages<T, L, NB, S + 1, TW32>(v, lds, tw64, t, tw);
     }
 }
 
@@ -295,7 +304,8 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
     using B = bigrow<T, L, NB>;
     constexpr int NT = B::NT, RL = B::RL, NSL = B::NSL, HALF = B::HALF;
 
-    radix16_stages<T, L, NB, 0>(v, lds, tw64, t);
+    constexpr bool TW32 = is16<SO>();   // 16-bit storage: FP32 twiddle products
+    radix16_stages<T, L, NB, 0, TW32>(v, lds, tw64, t, tw);
     {
         // last radix-16 stage; its outputs are handed to the fused stage through LDS
         constexpr int NS = B::ns(B::NR16 - 1);
@@ -303,7 +313,11 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
         for (int b = 0; b < NB; ++b)
         {
             const int k = (t + b * NT) & (NS - 1);
-            if constexpr (NS > 1) twiddle16<T, L, NS>(v + 16 * b, tw64, k);
+            if constexpr (NS > 1)
+            {
+                if constexpr (TW32) twiddle16<T, L, NS, T>(v + 16 * b, tw, k);
+                else twiddle16<T, L, NS>(v + 16 * b, tw64, k);
+            }
             dft16(v + 16 * b);
         }
     }
