@@ -14,7 +14,9 @@ Prints ONE JSON line.  Flags: --steps K --warmup W --gpus N (N must be 1)
 legs) --nx/--ny (other sizes) --copy-probe BYTES (only a device copy, for
 calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE) --general-only (only the
 general-length legs, one JSON line) --only LEG (one leg by name, for
-profiling it alone).
+profiling it alone).  The vendor_rocfft leg times hipFFT (over rocFFT),
+called directly and through torch.fft.rfft2, on the same transforms beside
+this build's plans.
 
 Every leg reports its launches (event-timed seconds, algorithmic bytes and,
 where a committed counter summary covers the workload, HBM traffic) and a
@@ -467,6 +469,86 @@ def cpu_baseline(nx: int, ny: int, budget_s: float = 20.0):
             "seconds": t, "phases": dict(loop.measurements)}
 
 
+def _hipfft_lib():
+    """torch's own hipFFT (the library torch.fft calls, over rocFFT), loaded
+    for direct calls: plans and executes without torch's tensor glue."""
+    import ctypes
+    import torch
+    lib = ctypes.CDLL(os.path.join(os.path.dirname(torch.__file__), "lib", "libhipfft.so"))
+    vp = ctypes.c_void_p
+    lib.hipfftPlan2d.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.hipfftSetStream.argtypes = [vp, vp]
+    lib.hipfftExecD2Z.argtypes = [vp, vp, vp]
+    lib.hipfftExecR2C.argtypes = [vp, vp, vp]
+    lib.hipfftDestroy.argtypes = [vp]
+    for f in (lib.hipfftPlan2d, lib.hipfftSetStream, lib.hipfftExecD2Z, lib.hipfftExecR2C, lib.hipfftDestroy):
+        f.restype = ctypes.c_int
+    return lib
+
+
+def bench_vendor(reps: int = 10) -> dict:
+    """The same transforms through the vendor FFT library on the same GPU and
+    stream, timed beside this build's plans: hipFFT (over rocFFT) called
+    directly -- hipfftPlan2d + hipfftExecD2Z / hipfftExecR2C -- and, as a
+    user would call it, torch.fft.rfft2 (the same library plus torch's copies).
+    A runtime comparator only: neither the implementation nor the parity
+    oracle (the rel-L2 between the outputs is reported, not tested).
+    Layout: the unpadded one both libraries use (input Nx x Ny real, output
+    Nx x (Ny/2+1) complex, out of place); bytes on the 2-pass model.  Ours
+    plans at effort measure, as the headline does; the vendor plans are made
+    before the timed calls."""
+    import ctypes
+    import torch
+    import hpxfft_amd
+
+    HIPFFT_R2C, HIPFFT_D2Z = 0x2A, 0x6A
+    hf = _hipfft_lib()
+    res = {"library": "hipFFT / rocFFT bundled with torch " + torch.__version__}
+    stream = torch.cuda.current_stream()
+    for nx, ny, prec in [(1 << 14, 1 << 14, "f64"), (1 << 14, 1 << 14, "f32"), (1 << 15, 1 << 15, "f32")]:
+        dt = torch.float64 if prec == "f64" else torch.float32
+        g = torch.Generator(device="cuda")
+        g.manual_seed(2)
+        x = torch.empty((nx, ny), dtype=dt, device="cuda").uniform_(-1.0, 1.0, generator=g)
+        yv = torch.empty((nx, ny // 2 + 1), dtype=torch.complex128 if prec == "f64" else torch.complex64,
+                         device="cuda")
+        yo = torch.empty_like(yv)
+        plan = hpxfft_amd.Plan(nx, ny, ld_in=ny, precision=prec, effort="measure", stream=stream)
+        oms = _events_ms(lambda: plan.exec(x.data_ptr(), yo.data_ptr()), reps)
+        plan.close()
+        h = ctypes.c_void_p()
+        t0 = time.perf_counter()
+        if hf.hipfftPlan2d(ctypes.byref(h), nx, ny, HIPFFT_D2Z if prec == "f64" else HIPFFT_R2C) != 0:
+            raise RuntimeError("hipfftPlan2d failed")
+        if hf.hipfftSetStream(h, ctypes.c_void_p(stream.cuda_stream)) != 0:
+            raise RuntimeError("hipfftSetStream failed")
+        ex = hf.hipfftExecD2Z if prec == "f64" else hf.hipfftExecR2C
+
+        def run_vendor():
+            if ex(h, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(yv.data_ptr())) != 0:
+                raise RuntimeError("hipfftExec failed")
+
+        run_vendor()
+        torch.cuda.synchronize()
+        vplan_s = time.perf_counter() - t0
+        diff = (torch.linalg.vector_norm((yo - yv).to(torch.complex128)) /
+                torch.linalg.vector_norm(yo.to(torch.complex128))).item()
+        vms = _events_ms(run_vendor, reps)
+        hf.hipfftDestroy(h)
+        torch.fft.rfft2(x, out=yv)
+        tms = _events_ms(lambda: torch.fft.rfft2(x, out=yv), reps)
+        b = alg_bytes(nx, ny, 8 if prec == "f64" else 4)["total"]
+        res[f"{prec}_{nx}x{ny}"] = {
+            "ours_ms": round(oms, 4), "hipfft_ms": round(vms, 4), "torch_rfft2_ms": round(tms, 4),
+            "ours_GBps": round(b / (oms * 1e-3) / 1e9, 1), "hipfft_GBps": round(b / (vms * 1e-3) / 1e9, 1),
+            "speedup_vs_hipfft": round(vms / oms, 3), "speedup_vs_torch_rfft2": round(tms / oms, 3),
+            "rel_l2_between": diff, "hipfft_plan_and_first_call_s": round(vplan_s, 3)}
+        progress(f"vendor {prec} {nx}x{ny}: hipFFT {vms:.4f} ms, torch.fft.rfft2 {tms:.4f} ms, ours {oms:.4f} ms")
+        del x, yv, yo
+        torch.cuda.empty_cache()
+    return res
+
+
 def _headline_leg(precision: str, steps: int = 5, warmup: int = 2) -> dict:
     n = 1 << 14
     r = time_gpu(n, n, precision, steps, warmup, seed=2)
@@ -490,6 +572,7 @@ LEGS = {
     "general_length_bluestein": lambda: bench_general(2039, 2 * 2039),
     "general_length_long": lambda: bench_general(6000, 12000),
     "general_length_two_level": lambda: bench_general(9000, 18000),
+    "vendor_rocfft": bench_vendor,
 }
 
 
@@ -623,6 +706,9 @@ def main() -> int:
         torch.cuda.empty_cache()
         result.update(general_legs())
         progress("general length done")
+        torch.cuda.empty_cache()
+        result["vendor_rocfft"] = bench_vendor()
+        progress("vendor comparator done")
         torch.cuda.empty_cache()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(nx, ny)
