@@ -124,6 +124,8 @@ struct dev_tables
 {
     double *tw64 = nullptr;   // complex double pairs
     float *tw32 = nullptr;    // complex float pairs
+    double *stw64 = nullptr;  // radix-16 stage twiddles of the long-row kernels (stage_tw_off layout)
+    float *stw32 = nullptr;
 };
 
 // The tables live in static device memory: one copy per device, filled from
@@ -132,6 +134,10 @@ struct dev_tables
 constexpr size_t TW_ENTRIES = size_t(2) << TW_MAX_LG;   // entries [N, 2N) for N = 1 .. 2^TW_MAX_LG
 __device__ __align__(256) double tw64_table[2 * TW_ENTRIES];
 __device__ __align__(256) float tw32_table[2 * TW_ENTRIES];
+// the long-row kernels' stage tables: for NS = 16 and 256, the values w_{16 NS}^{m k},
+// m = 1, 2, 4, 8, k < NS, as four arrays (entries of the tables above, regrouped)
+__device__ __align__(256) double stw64_table[2 * hpxfft_dev::STAGE_TW_ENTRIES];
+__device__ __align__(256) float stw32_table[2 * hpxfft_dev::STAGE_TW_ENTRIES];
 
 std::mutex g_tab_mu;
 std::map<int, dev_tables> g_tables;
@@ -163,11 +169,32 @@ int get_tables(int dev, dev_tables &out)
             h32[2 * idx + 1] = (float)h64[2 * idx + 1];
         }
     }
+    constexpr int SE = hpxfft_dev::STAGE_TW_ENTRIES;
+    std::vector<double> s64(2 * SE);
+    std::vector<float> s32(2 * SE);
+    for (const int NS : {16, 256})
+    {
+        const int off = NS == 16 ? hpxfft_dev::stage_tw_off<16>() : hpxfft_dev::stage_tw_off<256>();
+        const uint64_t N = 16 * (uint64_t)NS;
+        for (int q = 0; q < 4; ++q)
+            for (int k = 0; k < NS; ++k)
+            {
+                const size_t src = N + ((uint64_t)(1 << q) * k) % N, dst = off + q * NS + k;   // w_N^{2^q k}
+                s64[2 * dst] = h64[2 * src];
+                s64[2 * dst + 1] = h64[2 * src + 1];
+                s32[2 * dst] = h32[2 * src];
+                s32[2 * dst + 1] = h32[2 * src + 1];
+            }
+    }
     dev_tables t;
     HIPCHK(hipGetSymbolAddress(reinterpret_cast<void **>(&t.tw64), HIP_SYMBOL(tw64_table)));
     HIPCHK(hipGetSymbolAddress(reinterpret_cast<void **>(&t.tw32), HIP_SYMBOL(tw32_table)));
+    HIPCHK(hipGetSymbolAddress(reinterpret_cast<void **>(&t.stw64), HIP_SYMBOL(stw64_table)));
+    HIPCHK(hipGetSymbolAddress(reinterpret_cast<void **>(&t.stw32), HIP_SYMBOL(stw32_table)));
     HIPCHK(hipMemcpy(t.tw64, h64.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(t.tw32, h32.data(), sizeof(float) * 2 * n, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t.stw64, s64.data(), sizeof(double) * 2 * SE, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(t.stw32, s32.data(), sizeof(float) * 2 * SE, hipMemcpyHostToDevice));
     g_tables[dev] = t;
     out = t;
     return HPXFFT_OK;
@@ -1379,6 +1406,8 @@ int exec_forward(hpxfft_plan_s *p, const void *d_in, void *d_out, const int *in_
     r.nrows = (int)p->nx;
     r.tw = tw_ptr(p, p->ny);
     r.tw64 = tw64_ptr(p, p->ny);
+    r.stw64 = p->tabs.stw64;
+    r.stw32 = p->tabs.stw32;
     r.in_exp = s16 ? in_exp : nullptr;
     r.out_exp = s16 ? p->mid_exp : nullptr;
     r.exp_dist = (long long)p->nx;
@@ -1666,6 +1695,8 @@ int tune_rows(hpxfft_plan_s *p)
     r.nrows = (int)p->nx;
     r.tw = tw_ptr(p, p->ny);
     r.tw64 = tw64_ptr(p, p->ny);
+    r.stw64 = p->tabs.stw64;
+    r.stw32 = p->tabs.stw32;
     r.out_exp = s16 ? p->mid_exp : nullptr;
     r.exp_dist = (long long)p->nx;
     const int REPS = tune_reps(p->effort);
@@ -2297,6 +2328,8 @@ int exec_backward(hpxfft_plan_s *plan, const void *d_spec, void *d_real, bool ha
     r.nrows = (int)plan->nx;
     r.tw = tw_ptr(plan, plan->ny);
     r.tw64 = tw64_ptr(plan, plan->ny);
+    r.stw64 = plan->tabs.stw64;
+    r.stw32 = plan->tabs.stw32;
     if (plan->g2r.on)
     {
         if (int rc = run_rows_2level(plan, nullptr, d_real, plan->adj_work, nullptr, true, halve)) return rc;

@@ -65,6 +65,8 @@ struct RowArgs
     int nrows;
     const void *tw;        // w_{2L}^m, m < 2L  (compute precision)
     const void *tw64;      // the same table in FP64 (twiddle products of FP32 rows are formed in FP64)
+    const void *stw64;     // radix-16 stage twiddles of the long-row kernels (stage_tw_off), FP64
+    const void *stw32;     // the same in FP32
     const int *in_exp;     // optional per-row input exponent (value = stored * 2^-e)
     int *out_exp;          // 16-bit outputs: per-row exponent written here
     long long exp_dist;    // between grids, for in_exp/out_exp

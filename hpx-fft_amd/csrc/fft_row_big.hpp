@@ -49,17 +49,32 @@ template <typename T, int L, int NB = 1> struct bigrow
 template <typename T, int LG> constexpr int big_nb() { return sizeof(T) == 4 && LG >= 13 ? 2 : 1; }
 template <typename T, int LG> constexpr int big_min_waves() { return big_nb<T, LG>() == 2 ? 4 : 1; }
 
-// stage twiddles w_{16 NS}^{r k}, r = 1..15, from the w_{2L} table: table
-// values w, w^2, w^4, w^8 and products of at most 3 factors.  W = double: the
-// FP64 table, products formed in FP64 and (for FP32 rows) rounded once -- the
-// only FP64 step of the FP32 path.  W = T = float (rows stored in 16 bits):
-// the FP32 table, products in FP32 (a few FP32 ulps, far below the 16-bit
-// rounding of the outputs) at half the issue cost of the FP64 products.
-template <typename T, int L, int NS, typename W = double>
-__device__ __forceinline__ void twiddle16(cpx<T> *v, const cpx<W> *__restrict__ tab, int k)
+// Stage twiddle tables (RowArgs::stw64 / stw32): for each radix-16 stage
+// with twiddles, NS = 16 and 256 butterflies per group, four arrays of NS
+// entries, w_{16 NS}^{k}, w_{16 NS}^{2k}, w_{16 NS}^{4k}, w_{16 NS}^{8k}
+// (k < NS), at stage_tw_off<NS>() entries.  The values are those of the
+// w_{2L} table at index m k L / (8 NS); laid out this way, the butterflies
+// of a wave (consecutive k) read consecutive entries of 17 KiB of hot table,
+// where gathers from the 2L-entry table touch a line per lane and, at
+// L = 2^14 (a 512 KiB FP64 table), miss in L2 behind the streaming rows.
+constexpr int STAGE_TW_ENTRIES = 4 * 16 + 4 * 256;
+template <int NS> constexpr int stage_tw_off()
 {
-    const int base = k * (L / (8 * NS));   // index of w_{16 NS}^k in the w_{2L} table
-    const cpx<W> w1 = tab[base], w2 = tab[2 * base], w4 = tab[4 * base], w8 = tab[8 * base];
+    static_assert(NS == 16 || NS == 256, "stage tables cover NS = 16 and 256");
+    return NS == 16 ? 0 : 4 * 16;
+}
+
+// stage twiddles w_{16 NS}^{r k}, r = 1..15: table values w, w^2, w^4, w^8
+// and products of at most 3 factors.  W = double: the FP64 tables, products
+// formed in FP64 and (for FP32 rows) rounded once -- the only FP64 step of the
+// FP32 path.  W = T = float (rows stored in 16 bits): the FP32 tables,
+// products in FP32 (a few FP32 ulps, far below the 16-bit rounding of the
+// outputs) at half the issue cost of the FP64 products.
+template <typename T, int NS, typename W = double>
+__device__ __forceinline__ void twiddle16(cpx<T> *v, const cpx<W> *__restrict__ stab, int k)
+{
+    const cpx<W> *s = stab + stage_tw_off<NS>();
+    const cpx<W> w1 = s[k], w2 = s[NS + k], w4 = s[2 * NS + k], w8 = s[3 * NS + k];
     const cpx<W> w3 = cmul(w2, w1), w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
     // each factor is applied as soon as it is formed (fewer live registers)
     auto app = [&](int r, const cpx<W> &w) { v[r] = cmul(v[r], cpx<T>{(T)w.x, (T)w.y}); };
@@ -80,10 +95,11 @@ __device__ __forceinline__ void twiddle16(cpx<T> *v, const cpx<W> *__restrict__ 
     app(15, cmul(w8, w7));
 }
 
-// TW32: stage twiddles from the FP32 table tw (16-bit storage), else from tw64
+// TW32: stage twiddles from the FP32 stage tables stw32 (16-bit storage),
+// else from the FP64 tables stw64
 template <typename T, int L, int NB, int S, bool TW32 = false>
-__device__ __forceinline__ void radix16_stages(cpx<T> *v, T *lds, const cpx<double> *__restrict__ tw64, int t,
-                                               const cpx<T> *__restrict__ tw = nullptr)
+__device__ __forceinline__ void radix16_stages(cpx<T> *v, T *lds, const cpx<double> *__restrict__ stw64, int t,
+                                               const cpx<T> *__restrict__ stw32 = nullptr)
 {
     using B = bigrow<T, L, NB>;
     // stages 0 .. NR16-2 (each followed by an exchange); stage NR16-1 is
@@ -101,8 +117,8 @@ __device__ __forceinline__ void radix16_stages(cpx<T> *v, T *lds, const cpx<doub
             const int j = t + b * B::NT, k = j & (NS - 1);
             if constexpr (NS > 1)
             {
-                if constexpr (TW32) twiddle16<T, L, NS, T>(v + 16 * b, tw, k);
-                else twiddle16<T, L, NS>(v + 16 * b, tw64, k);
+                if constexpr (TW32) twiddle16<T, NS, T>(v + 16 * b, stw32, k);
+                else twiddle16<T, NS>(v + 16 * b, stw64, k);
             }
             dft16(v + 16 * b);
             wb[b] = B::pad((j / NS) * 16 * NS + k);
@@ -145,7 +161,7 @@ __device__ __forceinline__ void radix16_stages(cpx<T> *v, T *lds, const cpx<doub
                 for (int r = 0; r < 16; ++r) v[16 * b + r].y = lds[rb[b] + r * RS];
             __syncthreads();
         }
-        radix16_stages<T, L, NB, S + 1, TW32>(v, lds, tw64, t, tw);
+        radix16_stages<T, L, NB, S + 1, TW32>(v, lds, stw64, t, stw32);
     }
 }
 
@@ -297,7 +313,8 @@ __device__ __forceinline__ float cmax(cpx<T> a, cpx<T> b)
 // workgroup slot cleared before the first barrier, oexp the row's exponent.
 template <typename T, typename SO, int LG, int NB>
 __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__restrict__ tw,
-                                             const cpx<double> *__restrict__ tw64, SO *orow, int t,
+                                             const cpx<double> *__restrict__ stw64,
+                                             const cpx<T> *__restrict__ stw32, SO *orow, int t,
                                              unsigned *smax, int *oexp)
 {
     constexpr int L = 1 << LG;
@@ -305,7 +322,7 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
     constexpr int NT = B::NT, RL = B::RL, NSL = B::NSL, HALF = B::HALF;
 
     constexpr bool TW32 = is16<SO>();   // 16-bit storage: FP32 twiddle products
-    radix16_stages<T, L, NB, 0, TW32>(v, lds, tw64, t, tw);
+    radix16_stages<T, L, NB, 0, TW32>(v, lds, stw64, t, stw32);
     {
         // last radix-16 stage; its outputs are handed to the fused stage through LDS
         constexpr int NS = B::ns(B::NR16 - 1);
@@ -315,8 +332,8 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
             const int k = (t + b * NT) & (NS - 1);
             if constexpr (NS > 1)
             {
-                if constexpr (TW32) twiddle16<T, L, NS, T>(v + 16 * b, tw, k);
-                else twiddle16<T, L, NS>(v + 16 * b, tw64, k);
+                if constexpr (TW32) twiddle16<T, NS, T>(v + 16 * b, stw32, k);
+                else twiddle16<T, NS>(v + 16 * b, stw64, k);
             }
             dft16(v + 16 * b);
         }
@@ -499,7 +516,8 @@ row_r2c_big_kernel(RowArgs a)
     SO *orow = reinterpret_cast<SO *>(a.out) + (long long)b * a.dist_out + (long long)row * a.ld_out;
     int *oexp = (is16<SO>() && a.out_exp) ? a.out_exp + (long long)b * a.exp_dist + row : nullptr;
     big_row_body<T, SO, LG, NB>(v, lds, reinterpret_cast<const cpx<T> *>(a.tw),
-                            reinterpret_cast<const cpx<double> *>(a.tw64), orow, t, &smax, oexp);
+                                reinterpret_cast<const cpx<double> *>(a.stw64),
+                                reinterpret_cast<const cpx<T> *>(a.stw32), orow, t, &smax, oexp);
 }
 
 // Backward rows of 2^11, 2^13 and 2^14 complex points on the radix-16 stages
@@ -538,7 +556,7 @@ row_c2r_big_kernel(RowArgs a)
     const S *irow = reinterpret_cast<const S *>(a.in) + (long long)bat * a.dist_in + (long long)row * a.ld_in;
     S *orow = reinterpret_cast<S *>(a.out) + (long long)bat * a.dist_out + (long long)row * a.ld_out;
     const cpx<T> *tw = reinterpret_cast<const cpx<T> *>(a.tw);
-    const cpx<double> *tw64 = reinterpret_cast<const cpx<double> *>(a.tw64);
+    const cpx<double> *stw64 = reinterpret_cast<const cpx<double> *>(a.stw64);
 
     // pair tasks in chunks of CH: each chunk's loads of G[k], G[L-k] and the
     // twiddle are issued together (tasks past L/2 read the pair L/2 and write
@@ -609,7 +627,7 @@ row_c2r_big_kernel(RowArgs a)
         }
         __syncthreads();
     }
-    radix16_stages<T, L, NB, 0>(v, lds, tw64, t);
+    radix16_stages<T, L, NB, 0>(v, lds, stw64, t);
     constexpr int NS = B::ns(B::NR16 - 1);
     constexpr int WS = NS == 1 ? 1 : NS + NS / 16;
     constexpr int RS = NSL + NSL / 16;
@@ -617,7 +635,7 @@ row_c2r_big_kernel(RowArgs a)
     for (int b = 0; b < NB; ++b)
     {
         const int k = (t + b * NT) & (NS - 1);
-        if constexpr (NS > 1) twiddle16<T, L, NS>(v + 16 * b, tw64, k);
+        if constexpr (NS > 1) twiddle16<T, NS>(v + 16 * b, stw64, k);
         dft16(v + 16 * b);
     }
     // the last radix-16 stage's outputs to LDS; the radix-RL stage reads d[j + r NSL]
