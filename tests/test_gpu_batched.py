@@ -85,6 +85,32 @@ def test_batched_every_configuration(gpu, cfg):
         assert oracle.rel_l2(got, oracle.r2c_2d(xs[i], b)) <= 1e-5, (cfg, a, b)
 
 
+@pytest.mark.parametrize("nx,ny,tasks", [([64], [64], 1), ([64, 128, 64], [64, 64, 256], 3), ([2048], [128], 9)])
+def test_batched_fewer_column_tasks_than_ranges(gpu, nx, ny, tasks):
+    """The column tasks are cut into 8 contiguous ranges, one per XCD, each
+    with its own ticket counter; with fewer tasks than ranges most ranges are
+    empty and every workgroup takes its tasks from the others.  1, 3 and 9
+    column tasks (16384-point task shape), against the oracle."""
+    import re
+    import torch
+    import hpxfft_amd
+    xs = [oracle.fill(a, b, seed=5000 + i, dist="upm1") for i, (a, b) in enumerate(zip(nx, ny))]
+    ds = [torch.from_numpy(x).to(device=gpu, dtype=torch.float32) for x in xs]
+    plan = hpxfft_amd.BatchedPlan(nx, ny)
+    plan.set_config(2, 3)
+    col = [ln for ln in plan.describe().splitlines() if "column phase" in ln][0]
+    assert int(re.search(r"(\d+) tasks", col).group(1)) == tasks, col
+    for _ in range(2):   # the counters are cleared for every exec
+        for d, x in zip(ds, xs):
+            d.copy_(torch.from_numpy(x))
+        plan.exec([d.data_ptr() for d in ds])
+        torch.cuda.synchronize()
+        for i, (a, b) in enumerate(zip(nx, ny)):
+            got = ds[i].cpu().numpy().astype(np.float64).view(np.complex128)
+            assert oracle.rel_l2(got, oracle.r2c_2d(xs[i], b)) <= 1e-5, (a, b)
+    plan.close()
+
+
 def test_batched_set_config_rejections(gpu):
     import hpxfft_amd
     plan = hpxfft_amd.BatchedPlan([64], [64])
