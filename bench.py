@@ -16,7 +16,8 @@ calibrating rocprofv3 FETCH_SIZE / WRITE_SIZE) --general-only (only the
 general-length legs, one JSON line) --only LEG (one leg by name, for
 profiling it alone).  The vendor_rocfft leg times hipFFT (over rocFFT),
 called directly and through torch.fft.rfft2, on the same transforms beside
-this build's plans.
+this build's plans; cpu_library times scipy.fft.rfft2 on the host cores beside
+the oracle's CPU port (cpu_baseline).
 
 Every leg reports its launches (event-timed seconds, algorithmic bytes and,
 where a committed counter summary covers the workload, HBM traffic) and a
@@ -469,6 +470,30 @@ def cpu_baseline(nx: int, ny: int, budget_s: float = 20.0):
             "seconds": t, "phases": dict(loop.measurements)}
 
 
+def cpu_library(nx: int, ny: int, reps: int = 3) -> dict:
+    """A tuned CPU FFT library on the same host cores, beside the oracle's
+    port: scipy.fft.rfft2 (pocketfft) with one worker per affinity core, on
+    the same FP64 grid (unpadded Nx x Ny input, Nx x (Ny/2+1) output; bytes on
+    the 2-pass model).  Context for the port's figure, not the baseline."""
+    import numpy as np
+    import scipy
+    import scipy.fft
+
+    threads = len(os.sched_getaffinity(0))
+    x = np.random.default_rng(2).uniform(-1.0, 1.0, (nx, ny))
+    scipy.fft.rfft2(x, workers=threads)   # warm-up
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        scipy.fft.rfft2(x, workers=threads)
+        ts.append(time.perf_counter() - t0)
+    t = sorted(ts)[len(ts) // 2]
+    B = alg_bytes(nx, ny, 8)["total"]
+    return {"value": B / t / 1e9, "unit": "GB/s", "cores": threads, "seconds": t,
+            "library": f"scipy.fft.rfft2 (pocketfft, scipy {scipy.__version__}), workers={threads}",
+            "sample": f"full {nx}x{ny} FP64 grid, median of {reps} after 1 warm-up"}
+
+
 def _hipfft_lib():
     """torch's own hipFFT (the library torch.fft calls, over rocFFT), loaded
     for direct calls: plans and executes without torch's tensor glue."""
@@ -573,6 +598,7 @@ LEGS = {
     "general_length_long": lambda: bench_general(6000, 12000),
     "general_length_two_level": lambda: bench_general(9000, 18000),
     "vendor_rocfft": bench_vendor,
+    "cpu_library": lambda: cpu_library(1 << 14, 1 << 14),
 }
 
 
@@ -716,6 +742,11 @@ def main() -> int:
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(nx, ny)
         progress("cpu baseline done")
+        try:
+            result["cpu_library"] = cpu_library(nx, ny)
+        except Exception as e:   # an external library: its absence must not cost the bench line
+            result["cpu_library"] = {"error": f"{type(e).__name__}: {e}"}
+        progress("cpu library comparator done")
     print(json.dumps(result), flush=True)
     return 0
 
