@@ -224,6 +224,32 @@ template <typename S, typename T> __device__ __forceinline__ void st_stream(S *p
     }
     else stor<S>::st(p, z);
 }
+// The same for one lane of a wave whose 64 lanes access 64 consecutive
+// complex elements (DIR = +1: lane l at element e0 + l; DIR = -1: e0 - l).
+// Rows start at any 16-byte offset of a 128-byte line, so such a wave access
+// begins and ends inside lines it shares with the neighbouring access: those
+// lanes take the plain access (the L2 merges the two halves of the line),
+// the lanes of whole lines take the streaming one.  Only the cache hint
+// depends on the test; the value loaded or stored is the same either way.
+template <typename S, int DIR>
+__device__ __forceinline__ bool in_whole_line(const S *p, int lane)
+{
+    constexpr long long ES = 2 * (long long)sizeof(S);   // bytes per complex element
+    const long long a = (long long)reinterpret_cast<uintptr_t>(p);
+    const long long lo = DIR > 0 ? a - lane * ES : a - (63 - lane) * ES;   // the wave's byte range [lo, lo + 64 ES)
+    const long long line = a & ~127LL;
+    return line >= lo && line + 128 <= lo + 64 * ES;
+}
+template <typename S, int DIR = 1> __device__ __forceinline__ cpx<S> ld_stream_lines(const S *p, int lane)
+{
+    if (in_whole_line<S, DIR>(p, lane)) return ld_stream<S>(p);
+    return stor<S>::ld(p);
+}
+template <typename S, int DIR, typename T> __device__ __forceinline__ void st_stream_lines(S *p, cpx<T> z, int lane)
+{
+    if (in_whole_line<S, DIR>(p, lane)) st_stream<S>(p, z);
+    else stor<S>::st(p, z);
+}
 template <> struct stor<__half>
 {
     __device__ static cpx<float> ld(const __half *p)

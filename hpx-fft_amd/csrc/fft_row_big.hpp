@@ -382,8 +382,8 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
                     r2c_val(a[i][q], c[i][RL - 1 - q], split_twiddle<T, L, RL>(wj, j, q, tw), x0, x1);
                     if constexpr (sizeof(SO) == 8)   // FP64: streaming stores, as in r2c_pair
                     {
-                        st_stream<SO>(orow + 2 * (j + q * NSL), x0);
-                        st_stream<SO>(orow + 2 * (L - j - q * NSL), x1);
+                        st_stream_lines<SO, 1>(orow + 2 * (j + q * NSL), x0, t & 63);
+                        st_stream_lines<SO, -1>(orow + 2 * (L - j - q * NSL), x1, t & 63);
                     }
                     else
                     {
@@ -497,11 +497,16 @@ row_r2c_big_kernel(RowArgs a)
     cpx<T> v[16 * NB];
 #pragma unroll
     // FP32 / FP64 rows: streaming loads of the input (read once; measured
-    // 6% faster for FP32 rows, 0.5% over the three FP64 launches)
+    // 6% faster for FP32 rows, 0.5% over the three FP64 launches).  FP64 rows
+    // (and their streaming stores) keep the hint only on lanes whose 128-byte
+    // line lies inside the wave's access: their fetch and write counts fell
+    // to the algorithmic bytes and the 2^14 x 2^14 launch from 0.78 to
+    // 0.75 ms; FP32 rows measured slower with the split (0.41 -> 0.49 ms)
     for (int u = 0; u < 16 * NB; ++u)
     {
         const SI *src = irow + 2 * (t + (u / 16) * NT + (u % 16) * B::M);
-        if constexpr (sizeof(SI) >= 4) v[u] = ld_stream<SI>(src);
+        if constexpr (sizeof(SI) == 8) v[u] = ld_stream_lines<SI>(src, t & 63);   // FP64: whole lines only
+        else if constexpr (sizeof(SI) == 4) v[u] = ld_stream<SI>(src);
         else v[u] = stor<SI>::ld(src);
     }
     if constexpr (is16<SI>())
