@@ -240,14 +240,15 @@ __device__ __forceinline__ bool in_whole_line(const S *p, int lane)
     const long long line = a & ~127LL;
     return line >= lo && line + 128 <= lo + 64 * ES;
 }
-template <typename S, int DIR = 1> __device__ __forceinline__ cpx<S> ld_stream_lines(const S *p, int lane)
+// streaming access where `whole` (in_whole_line of the lane), plain otherwise
+template <typename S> __device__ __forceinline__ cpx<S> ld_stream_if(const S *p, bool whole)
 {
-    if (in_whole_line<S, DIR>(p, lane)) return ld_stream<S>(p);
+    if (whole) return ld_stream<S>(p);
     return stor<S>::ld(p);
 }
-template <typename S, int DIR, typename T> __device__ __forceinline__ void st_stream_lines(S *p, cpx<T> z, int lane)
+template <typename S, typename T> __device__ __forceinline__ void st_stream_if(S *p, cpx<T> z, bool whole)
 {
-    if (in_whole_line<S, DIR>(p, lane)) st_stream<S>(p, z);
+    if (whole) st_stream<S>(p, z);
     else stor<S>::st(p, z);
 }
 template <> struct stor<__half>

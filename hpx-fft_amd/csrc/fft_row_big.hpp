@@ -353,7 +353,12 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
 
     if constexpr (!is16<SO>())
     {
-        // FP64 / FP32 storage: store as produced
+        // FP64 / FP32 storage: store as produced.  FP64: the streaming hint on
+        // the lanes of whole lines only; a thread's stores are NT or NSL
+        // elements apart (multiples of 128 bytes), so one test per direction
+        static_assert((NT * 2 * sizeof(SO)) % 128 == 0 && (NSL * 2 * sizeof(SO)) % 128 == 0, "line-aligned strides");
+        const bool wf = in_whole_line<SO, 1>(orow + 2 * t, t & 63);
+        const bool wb = in_whole_line<SO, -1>(orow + 2 * (L - t), t & 63);
 #pragma unroll
         for (int i = 0; i < B::NMAIN; ++i)
         {
@@ -382,8 +387,8 @@ __device__ __forceinline__ void big_row_body(cpx<T> *v, T *lds, const cpx<T> *__
                     r2c_val(a[i][q], c[i][RL - 1 - q], split_twiddle<T, L, RL>(wj, j, q, tw), x0, x1);
                     if constexpr (sizeof(SO) == 8)   // FP64: streaming stores, as in r2c_pair
                     {
-                        st_stream_lines<SO, 1>(orow + 2 * (j + q * NSL), x0, t & 63);
-                        st_stream_lines<SO, -1>(orow + 2 * (L - j - q * NSL), x1, t & 63);
+                        st_stream_if<SO>(orow + 2 * (j + q * NSL), x0, wf);
+                        st_stream_if<SO>(orow + 2 * (L - j - q * NSL), x1, wb);
                     }
                     else
                     {
@@ -495,18 +500,24 @@ row_r2c_big_kernel(RowArgs a)
     const int b = g / a.nrows, row = g - b * a.nrows;
     const SI *irow = reinterpret_cast<const SI *>(a.in) + (long long)b * a.dist_in + (long long)row * a.ld_in;
     cpx<T> v[16 * NB];
-#pragma unroll
     // FP32 / FP64 rows: streaming loads of the input (read once; measured
-    // 6% faster for FP32 rows, 0.5% over the three FP64 launches).  FP64 rows
-    // (and their streaming stores) keep the hint only on lanes whose 128-byte
-    // line lies inside the wave's access: their fetch and write counts fell
-    // to the algorithmic bytes and the 2^14 x 2^14 launch from 0.78 to
-    // 0.75 ms; FP32 rows measured slower with the split (0.41 -> 0.49 ms)
+    // 6% faster for FP32 rows, 0.5% over the three FP64 launches).  FP64
+    // rows take the hint only on the lanes whose 128-byte line lies inside
+    // the wave's access, and load the line it shares with its neighbour
+    // plainly: the row pass's fetch fell to its algorithmic bytes and the
+    // 2^14 x 2^14 launch from 0.78 to 0.74 ms.  A thread's loads are NT or M
+    // elements apart (multiples of 128 bytes), so one test holds for all of
+    // them.  FP32 rows keep the hint everywhere: at their 128-VGPR bound the
+    // test spilled registers and made them slower (0.41 -> 0.49 ms).
+    static_assert(sizeof(SI) != 8 || ((NT * 2 * sizeof(SI)) % 128 == 0 && (B::M * 2 * sizeof(SI)) % 128 == 0),
+                  "line-aligned strides");
+    bool whole = true;
+    if constexpr (sizeof(SI) == 8) whole = in_whole_line<SI, 1>(irow + 2 * t, t & 63);
+#pragma unroll
     for (int u = 0; u < 16 * NB; ++u)
     {
         const SI *src = irow + 2 * (t + (u / 16) * NT + (u % 16) * B::M);
-        if constexpr (sizeof(SI) == 8) v[u] = ld_stream_lines<SI>(src, t & 63);   // FP64: whole lines only
-        else if constexpr (sizeof(SI) == 4) v[u] = ld_stream<SI>(src);
+        if constexpr (sizeof(SI) >= 4) v[u] = ld_stream_if<SI>(src, whole);
         else v[u] = stor<SI>::ld(src);
     }
     if constexpr (is16<SI>())
