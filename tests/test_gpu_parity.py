@@ -136,6 +136,31 @@ def test_parity_batched_strided(gpu, nx, ny, batch, prec, tol):
         assert np.all(out[b * dist_out + nx * ld_out: (b + 1) * dist_out] == complex(7.0, -7.0)), b
 
 
+@pytest.mark.parametrize("ny", [4096, 16384, 32768])
+@pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
+def test_long_rows_every_line_offset(gpu, ny, prec, tol):
+    """Long rows (2^11, 2^13, 2^14 complex points) whose starts cycle
+    through every offset within a 128-byte line, in and out: input rows one
+    complex element longer than Ny reals (ld_in = Ny + 2), output rows Ny/2+1
+    complex, out of place.  The FP64 long-row kernel chooses its cache hint
+    per lane from that offset; the result must be the oracle's at every one."""
+    import torch
+    import hpxfft_amd
+    nx, ld_in, ncy = 16, ny + 2, ny // 2 + 1
+    x = oracle.fill(nx, ny, seed=ny + 11, dist="upm1", ld=ld_in)
+    dt = torch.float64 if prec == "f64" else torch.float32
+    cdt = torch.complex128 if prec == "f64" else torch.complex64
+    d_in = torch.from_numpy(x).to(device="cuda", dtype=dt).contiguous()
+    d_out = torch.empty((nx, ncy), dtype=cdt, device="cuda")
+    p = hpxfft_amd.Plan(nx, ny, ld_in=ld_in, ld_out=ncy, precision=prec)
+    p.exec(d_in.data_ptr(), d_out.data_ptr())
+    torch.cuda.synchronize()
+    p.close()
+    err = oracle.rel_l2(d_out.cpu().numpy().astype(np.complex128), oracle.r2c_2d(x, ny))
+    print(f"{prec} {nx}x{ny} ld_in {ld_in}: rel-L2 {err:.3e}", flush=True)
+    assert err <= tol, err
+
+
 @pytest.mark.parametrize("prec,tol", [("f64", TOL64), ("f32", TOL32)])
 @pytest.mark.parametrize("nx,ny", [(4096, 256), (2048, 64)])
 def test_measure_effort_autotunes_split(gpu, nx, ny, prec, tol):
